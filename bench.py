@@ -1,0 +1,188 @@
+"""Benchmark of the single-rank 3D FFT path on one MI355X.
+
+Workload (the headline configuration): c2c 512 x 512 x 512 complex<float>,
+input box order (0,1,2), output box order (1,2,0), in place with a
+caller-provided workspace, forward + backward with scale::symmetric, values iid
+N(0,1).  One step = one forward + backward pair, as heFFTe's speed3d harness
+times it (benchmarks/speed3d.h:162-191, 228-229).
+
+Prints one JSON line with ms per pair, 5 N log2 N GFLOP/s, the roofline of the
+dominant kernel (the 512-point pass; algorithmic bytes = read + write of the
+whole array per launch), the error against the FP64 path, the rocFFT time of
+the same transform (torch.fft on this GPU, for comparison) and the CPU baseline
+(the FP64 oracle on the host cores).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "512³ c2c fwd+bwd ms & %HBM roofline; split-FP16 vs FP32 at ≤5e-6 rel L2"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=512, help="cube side")
+    p.add_argument("--dtype", default="complex64", choices=["complex64", "complex128"])
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-rocfft", action="store_true", help="skip the rocFFT comparison")
+    p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
+    return p.parse_args()
+
+
+def cpu_baseline(n):
+    """The FP64 oracle (a CPU restatement of heFFTe's path) on the host cores:
+    one forward + backward pair of the same transform, full size."""
+    import numpy as np
+    from oracle import pyoracle
+    threads = min(16, os.cpu_count() or 1)
+    pyoracle.set_threads(threads)
+    if n > 512:
+        return None
+    m = n
+    b = pyoracle.Box.make((0, 0, 0), (m - 1, m - 1, m - 1), (0, 1, 2))
+    bo = pyoracle.Box.make((0, 0, 0), (m - 1, m - 1, m - 1), (1, 2, 0))
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(m ** 3) + 1j * rng.standard_normal(m ** 3)
+    s = 1.0 / math.sqrt(m ** 3)
+    t0 = time.perf_counter()
+    y = pyoracle.fft3d_c2c(b, bo, x, -1, s)
+    del x
+    z = pyoracle.fft3d_c2c(bo, b, y, +1, s)
+    t1 = time.perf_counter()
+    del y, z
+    return {"value": round((t1 - t0) * 1e3, 1), "unit": "ms per fwd+bwd pair", "cores": threads, "kind": "port",
+            "sample": f"one full {m}^3 forward+backward pair of the FP64 oracle (complex<double>, "
+                      f"OpenMP, {threads} threads), (0,1,2)->(1,2,0), scale::symmetric"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only (single-rank path); "
+                                   "multi-GPU runs are not part of this build", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import heffte_amd as heffte
+
+    dev = torch.device("cuda:0")
+    n = args.n
+    N = n ** 3
+    cdt = torch.complex64 if args.dtype == "complex64" else torch.complex128
+    esize = 8 if args.dtype == "complex64" else 16
+    inbox = heffte.box3d([0, 0, 0], [n - 1] * 3, [0, 1, 2])
+    outbox = heffte.box3d([0, 0, 0], [n - 1] * 3, [1, 2, 0])
+    plan = heffte.fft3d(heffte.backend.mi355, inbox, outbox)
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    data = torch.randn(N, dtype=cdt, device=dev, generator=g) * math.sqrt(2.0)  # re, im ~ N(0,1)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=cdt, device=dev)
+    x0 = data.clone() if not args.no_error else None
+
+    def step():
+        plan.forward_buffered(data, data, ws, heffte.scale.symmetric)
+        plan.backward_buffered(data, data, ws, heffte.scale.symmetric)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_total = ev0.elapsed_time(ev1)
+    ms_pair = ms_total / args.steps
+
+    # dominant kernel: every launch of a pair is the n-point pass (3 forward + 3 backward)
+    launches_per_pair = 6
+    avg_launch_ms = ms_pair / launches_per_pair
+    bytes_per_launch = 2.0 * N * esize  # read + write of the array
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    gflops = 2 * 5.0 * N * math.log2(N) / (ms_pair * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(ms_pair, 4),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_pair, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.dtype == "complex64" else "f64",
+        "data": "synthetic (iid N(0,1) real and imaginary parts, torch generator seed 1)",
+        "config": {"workload": f"c2c {n}x{n}x{n} {'complex<float>' if esize == 8 else 'complex<double>'}, "
+                               "inbox order (0,1,2) -> outbox order (1,2,0), in place, caller workspace, "
+                               "forward+backward scale::symmetric",
+                   "n": n, "batch": 1, "parallelism": "single rank, 1 GPU"},
+        "gflops_5nlog2n": round(gflops, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"fft_uniform {n}-point pass", "avg_launch_ms": round(avg_launch_ms, 4),
+                     "bytes_per_launch": bytes_per_launch},
+    }
+
+    if not args.no_error:
+        # error of one forward transform against the FP64 path (itself pinned to the oracle by the tests)
+        y = torch.empty_like(x0)
+        plan.forward_buffered(x0, y, ws, heffte.scale.none)
+        ws64 = torch.empty(max(1, plan.size_workspace()), dtype=torch.complex128, device=dev)
+        x64 = x0.to(torch.complex128)
+        del x0
+        y64 = torch.empty_like(x64)
+        plan.forward_buffered(x64, y64, ws64, heffte.scale.none)
+        torch.cuda.synchronize()
+        num = torch.linalg.vector_norm(y.to(torch.complex128) - y64).item()
+        den = torch.linalg.vector_norm(y64).item()
+        out["rel_l2_vs_fp64"] = num / den
+        del y, x64, y64, ws64
+        torch.cuda.empty_cache()
+
+    if not args.no_rocfft:
+        try:
+            x = torch.randn(n, n, n, dtype=cdt, device=dev, generator=g)
+            for _ in range(2):
+                torch.fft.ifftn(torch.fft.fftn(x, norm="ortho"), norm="ortho")
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = max(3, args.steps // 2)
+            e0.record()
+            for _ in range(reps):
+                torch.fft.ifftn(torch.fft.fftn(x, norm="ortho"), norm="ortho")
+            e1.record()
+            torch.cuda.synchronize()
+            out["rocfft_torch_fftn_ms_per_pair"] = round(e0.elapsed_time(e1) / reps, 4)
+            del x
+            torch.cuda.empty_cache()
+        except Exception as e:  # pragma: no cover
+            out["rocfft_torch_fftn_ms_per_pair"] = f"unavailable: {e}"
+
+    if not args.no_cpu:
+        del data, ws
+        torch.cuda.empty_cache()
+        out["cpu_baseline"] = cpu_baseline(n)
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,277 @@
+"""Python binding of libheffte_mi355.so, the MI355X single-rank 3D FFT backend.
+
+It mirrors heFFTe's ctypes binding (python/heffte.py: backend, scale, box3d,
+fft3d(), fft3d_r2c(), plan.forward/backward and their _buffered variants) with
+torch tensors on the GPU as the arrays.  torch is only the allocator here:
+every transform runs in the hand-written HIP kernels of the library, and there
+is no fallback -- if the library is missing, importing this module raises.
+
+    import torch, heffte_amd as heffte
+    box = heffte.box3d([0, 0, 0], [63, 63, 63])
+    out = heffte.box3d([0, 0, 0], [63, 63, 63], [2, 1, 0])
+    plan = heffte.fft3d(heffte.backend.mi355, box, out)
+    y = torch.empty(plan.size_outbox(), dtype=torch.complex128, device="cuda")
+    plan.forward(x, y, heffte.scale.none)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY_PATH = os.path.join(_HERE, "libheffte_mi355.so")
+
+
+class heffte_input_error(Exception):
+    """Raised on invalid input, as python/heffte.py:17-33 does."""
+
+    def __init__(self, message):
+        super().__init__(message)
+        self.message = message
+
+
+def _load():
+    if not os.path.exists(LIBRARY_PATH):
+        raise ImportError(
+            f"{LIBRARY_PATH} is missing: build it with `python heffte_amd/build.py` "
+            "(there is no CPU or PyTorch fallback for the transforms)")
+    lib = ctypes.CDLL(LIBRARY_PATH)
+    LL3 = ctypes.POINTER(ctypes.c_longlong)
+    I3 = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    plan_p = ctypes.c_void_p
+    lib.heffte_mi355_plan_create.argtypes = [LL3, LL3, I3, LL3, LL3, I3, vp, ctypes.POINTER(plan_p)]
+    lib.heffte_mi355_plan_create_r2c.argtypes = [LL3, LL3, I3, LL3, LL3, I3, ctypes.c_int, vp, ctypes.POINTER(plan_p)]
+    lib.heffte_mi355_plan_create_r2r.argtypes = [ctypes.c_int, LL3, LL3, I3, LL3, LL3, I3, vp, ctypes.POINTER(plan_p)]
+    lib.heffte_mi355_plan_destroy.argtypes = [plan_p]
+    for f in ("heffte_mi355_size_inbox", "heffte_mi355_size_outbox", "heffte_mi355_size_workspace"):
+        getattr(lib, f).argtypes = [plan_p]
+        getattr(lib, f).restype = ctypes.c_longlong
+    lib.heffte_mi355_get_kind.argtypes = [plan_p]
+    lib.heffte_mi355_set_stream.argtypes = [plan_p, vp]
+    for f in ("forward_c2c", "forward_z2z", "forward_s2c", "forward_d2z", "backward_c2c", "backward_z2z",
+              "backward_c2s", "backward_z2d", "forward_s2s", "forward_d2d", "backward_s2s", "backward_d2d"):
+        fn = getattr(lib, "heffte_mi355_" + f)
+        fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, ctypes.c_int]
+        fn.restype = ctypes.c_int
+    lib.heffte_mi355_reshape.argtypes = [ctypes.c_int, LL3, LL3, I3, I3, ctypes.c_int, vp, vp, vp]
+    lib.heffte_mi355_last_error.restype = ctypes.c_char_p
+    lib.heffte_mi355_set_default_options.argtypes = [vp]
+    return lib
+
+
+lib = _load()
+
+# exported symbols of include/heffte_mi355.h (checked by the tests)
+C_ABI_SYMBOLS = (
+    "heffte_mi355_set_default_options", "heffte_mi355_plan_create", "heffte_mi355_plan_create_r2c",
+    "heffte_mi355_plan_create_r2r", "heffte_mi355_plan_destroy", "heffte_mi355_size_inbox",
+    "heffte_mi355_size_outbox", "heffte_mi355_size_workspace", "heffte_mi355_get_kind", "heffte_mi355_set_stream",
+    "heffte_mi355_forward_c2c", "heffte_mi355_forward_z2z", "heffte_mi355_forward_s2c", "heffte_mi355_forward_d2z",
+    "heffte_mi355_backward_c2c", "heffte_mi355_backward_z2z", "heffte_mi355_backward_c2s",
+    "heffte_mi355_backward_z2d", "heffte_mi355_forward_s2s", "heffte_mi355_forward_d2d",
+    "heffte_mi355_backward_s2s", "heffte_mi355_backward_d2d", "heffte_mi355_reshape", "heffte_mi355_last_error",
+)
+
+
+class backend:
+    """Backend ids (python/heffte.py:110-116); this library provides mi355 only."""
+    mi355 = 12
+    valid = [12]
+
+
+class scale:
+    none = 0
+    full = 1
+    symmetric = 2
+
+
+class kind:
+    c2c = 0
+    r2c = 1
+    cos = 2
+    sin = 3
+
+
+class box3d:
+    """heffte::box3d: inclusive low/high indexes and the (fast, mid, slow) order."""
+
+    def __init__(self, clow, chigh, corder=(0, 1, 2)):
+        self.low = np.array(clow, dtype=np.int64)
+        self.high = np.array(chigh, dtype=np.int64)
+        self.order = np.array(corder, dtype=np.int32)
+        if self.low.size != 3 or self.high.size != 3 or self.order.size != 3:
+            raise heffte_input_error("box3d needs three low, high and order entries")
+        if sorted(self.order.tolist()) != [0, 1, 2]:
+            raise heffte_input_error("box3d order must be a permutation of (0, 1, 2)")
+
+    @property
+    def size(self):
+        return tuple(int(h - l + 1) for l, h in zip(self.low, self.high))
+
+    def count(self):
+        s = self.size
+        return 0 if min(s) <= 0 else s[0] * s[1] * s[2]
+
+    def osize(self, i):
+        return self.size[self.order[i]]
+
+    def r2c(self, dim):
+        """heffte_geometry.h:96-104."""
+        high = self.high.copy()
+        high[dim] = self.low[dim] + self.size[dim] // 2
+        return box3d(self.low, high, self.order)
+
+    def _c(self):
+        LL = ctypes.c_longlong * 3
+        I3 = ctypes.c_int * 3
+        return LL(*self.low.tolist()), LL(*self.high.tolist()), I3(*self.order.tolist())
+
+
+def last_error():
+    return lib.heffte_mi355_last_error().decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise heffte_input_error(f"{what} failed with code {rc}: {last_error()}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    raise heffte_input_error("arrays must be torch tensors on the GPU or raw device addresses")
+
+
+def _dtype_name(t):
+    return str(t.dtype).replace("torch.", "")
+
+
+class fft_plan:
+    """A plan object; create it with fft3d(), fft3d_r2c() or rtransform()."""
+
+    def __init__(self, handle, k):
+        self._h = handle
+        self.kind = k
+        self.use_r2c = (k == kind.r2c)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and lib is not None:
+            lib.heffte_mi355_plan_destroy(h)
+            self._h = None
+
+    def size_inbox(self):
+        return lib.heffte_mi355_size_inbox(self._h)
+
+    def size_outbox(self):
+        return lib.heffte_mi355_size_outbox(self._h)
+
+    def size_workspace(self):
+        return lib.heffte_mi355_size_workspace(self._h)
+
+    def set_stream(self, stream):
+        """stream: a torch.cuda.Stream, a raw hipStream_t address, or None (default stream)."""
+        s = getattr(stream, "cuda_stream", stream)
+        _check(lib.heffte_mi355_set_stream(self._h, ctypes.c_void_p(s) if s else None), "set_stream")
+
+    def _call(self, direction, inarray, outarray, workspace, scaling, batch):
+        if scaling not in (0, 1, 2):
+            raise heffte_input_error(f"{direction}() called with invalid scaling")
+        ti, to = _dtype_name(inarray), _dtype_name(outarray)
+        if self.kind in (kind.cos, kind.sin):
+            table = {("float32", "float32"): "s2s", ("float64", "float64"): "d2d"}
+        elif direction == "forward":
+            table = {("float32", "complex64"): "s2c", ("float64", "complex128"): "d2z"}
+            if not self.use_r2c:
+                table.update({("complex64", "complex64"): "c2c", ("complex128", "complex128"): "z2z"})
+        else:
+            table = {("complex64", "float32"): "c2s", ("complex128", "float64"): "z2d"}
+            if not self.use_r2c:
+                table.update({("complex64", "complex64"): "c2c", ("complex128", "complex128"): "z2z"})
+        key = table.get((ti, to))
+        if key is None:
+            raise heffte_input_error(f"{direction}() called with incompatible types {ti} -> {to}")
+        nin, nout = (self.size_inbox(), self.size_outbox()) if direction == "forward" else (
+            self.size_outbox(), self.size_inbox())
+        if inarray.numel() < batch * nin or outarray.numel() < batch * nout:
+            raise heffte_input_error(f"{direction}() called with invalid array size")
+        if workspace is not None and workspace.numel() * workspace.element_size() < \
+                batch * self.size_workspace() * (16 if key in ("z2z", "d2z", "z2d") else
+                                                 8 if key in ("c2c", "s2c", "c2s", "d2d") else 4):
+            raise heffte_input_error(f"{direction}() called with insufficient workspace size")
+        fn = getattr(lib, f"heffte_mi355_{direction}_{key}")
+        _check(fn(self._h, batch, _ptr(inarray), _ptr(outarray), _ptr(workspace), scaling), direction)
+
+    def forward(self, inarray, outarray, scaling=scale.none, batch=1):
+        self._call("forward", inarray, outarray, None, scaling, batch)
+
+    def backward(self, inarray, outarray, scaling=scale.none, batch=1):
+        self._call("backward", inarray, outarray, None, scaling, batch)
+
+    def forward_buffered(self, inarray, outarray, workspace, scaling=scale.none, batch=1):
+        self._call("forward", inarray, outarray, workspace, scaling, batch)
+
+    def backward_buffered(self, inarray, outarray, workspace, scaling=scale.none, batch=1):
+        self._call("backward", inarray, outarray, workspace, scaling, batch)
+
+
+def _check_backend(backend_tag):
+    if backend_tag not in backend.valid:
+        raise heffte_input_error("Invalid backend, use heffte_amd.backend.mi355")
+
+
+def _check_comm(comm):
+    if comm is not None:
+        size = comm.Get_size() if hasattr(comm, "Get_size") else int(comm)
+        if size != 1:
+            raise heffte_input_error("this backend runs one rank per transform (communicator of size 1)")
+
+
+def fft3d(backend_tag, inbox, outbox, comm=None):
+    """python/heffte.py fft3d(); comm may be None or any communicator of size 1."""
+    _check_backend(backend_tag)
+    _check_comm(comm)
+    h = ctypes.c_void_p()
+    il, ih, io = inbox._c()
+    ol, oh, oo = outbox._c()
+    _check(lib.heffte_mi355_plan_create(il, ih, io, ol, oh, oo, None, ctypes.byref(h)), "fft3d")
+    return fft_plan(h, kind.c2c)
+
+
+def fft3d_r2c(backend_tag, inbox, outbox, r2c_direction, comm=None):
+    _check_backend(backend_tag)
+    _check_comm(comm)
+    if r2c_direction not in (0, 1, 2):
+        raise heffte_input_error("fft3d_r2c() called with invalid r2c_direction, must use 0, 1, or 2")
+    h = ctypes.c_void_p()
+    il, ih, io = inbox._c()
+    ol, oh, oo = outbox._c()
+    _check(lib.heffte_mi355_plan_create_r2c(il, ih, io, ol, oh, oo, r2c_direction, None, ctypes.byref(h)),
+           "fft3d_r2c")
+    return fft_plan(h, kind.r2c)
+
+
+def rtransform(backend_tag, inbox, outbox, transform=kind.cos, comm=None):
+    """heffte::rtransform<backend::*_cos / *_sin>."""
+    _check_backend(backend_tag)
+    _check_comm(comm)
+    h = ctypes.c_void_p()
+    il, ih, io = inbox._c()
+    ol, oh, oo = outbox._c()
+    _check(lib.heffte_mi355_plan_create_r2r(transform, il, ih, io, ol, oh, oo, None, ctypes.byref(h)),
+           "rtransform")
+    return fft_plan(h, transform)
+
+
+def reshape(inbox, out_order, inarray, outarray, batch=1, stream=None):
+    """Bit-exact in-node reshape of inarray (stored in inbox.order) into out_order."""
+    il, ih, io = inbox._c()
+    oo = (ctypes.c_int * 3)(*[int(v) for v in out_order])
+    s = getattr(stream, "cuda_stream", stream)
+    _check(lib.heffte_mi355_reshape(inarray.element_size(), il, ih, io, oo, batch, _ptr(inarray), _ptr(outarray),
+                                    ctypes.c_void_p(s) if s else None), "reshape")

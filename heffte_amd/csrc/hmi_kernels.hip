@@ -1,0 +1,404 @@
+// gfx950 FFT pass kernels.
+//
+//  * fft_uniform<...>: compile-time radix plan, every thread owns E elements of
+//    one line for the whole transform; stages exchange data through a padded
+//    LDS tile.  The thread->element mapping used for the global load and the
+//    one used for the global store are chosen independently (walk the FFT axis
+//    or walk the column axis), which is how a layout permutation is fused into
+//    a pass with coalesced HBM traffic on both sides.
+//  * fft_generic<R>: runtime radix plan (any length whose tile fits in LDS,
+//    any prime factor), Stockham ping-pong in LDS.
+//  * permute_kernel: heFFTe's in-node reshape (transpose_unpacker,
+//    src/heffte_backend_rocm.hip.cpp:91-134) as an LDS-tiled permutation.
+//
+// Stockham autosort, stage s with radix R and Ls = product of earlier radices:
+//   butterfly b in [0, n/R): inputs x[b + p n/R], twiddles w_{Ls R}^{p (b mod Ls)},
+//   outputs to (b / Ls) Ls R + (b mod Ls) + q Ls.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "hmi_device.h"
+#include "hmi_kernels.h"
+
+namespace hmi {
+
+template <int... Rs> struct radices {
+    static constexpr int count = sizeof...(Rs);
+    static constexpr int at(int i) {
+        constexpr int a[] = {Rs...};
+        return a[i];
+    }
+    static constexpr int ls(int s) {
+        int p = 1;
+        for (int i = 0; i < s; i++) p *= at(i);
+        return p;
+    }
+    static constexpr int twoff(int s) {
+        int o = 0;
+        for (int i = 0; i < s; i++) o += ls(i) * (at(i) - 1);
+        return o;
+    }
+    static constexpr int n() { return ls(count); }
+};
+
+template <int PS> __device__ __forceinline__ int padj(int j) { return j + (j >> PS); }
+
+template <int S, typename RL, int E, int P, int RS, int PS, typename C>
+__device__ __forceinline__ void run_stages(C (&v)[E], C *lds, const C *__restrict__ tw, int l, int t, int l2,
+                                           int t2) {
+    constexpr int R = RL::at(S);
+    constexpr int Ls = RL::ls(S);
+    constexpr int NB = E / R;
+    constexpr bool last = (S + 1 == RL::count);
+    constexpr int TWO = RL::twoff(S);
+    static_assert(E % R == 0, "every radix must divide the elements per thread");
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        const int b = t + P * r;
+        const int k = b % Ls;
+        C x[R];
+#pragma unroll
+        for (int p = 0; p < R; p++) x[p] = v[r + NB * p];
+        if constexpr (Ls > 1) {
+            const C *w = tw + TWO + k * (R - 1);
+#pragma unroll
+            for (int p = 1; p < R; p++) x[p] = cmul(x[p], w[p - 1]);
+        }
+        dft<R>::run(x);
+        if constexpr (last) {
+#pragma unroll
+            for (int q = 0; q < R; q++) v[r + NB * q] = x[q];
+        } else {
+            const int base = (b / Ls) * Ls * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) lds[l * RS + padj<PS>(base + q * Ls)] = x[q];
+        }
+    }
+    if constexpr (!last) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; e++) v[e] = lds[l2 * RS + padj<PS>(t2 + P * e)];
+        __syncthreads();
+        run_stages<S + 1, RL, E, P, RS, PS>(v, lds, tw, l2, t2, l2, t2);
+    }
+}
+
+template <typename R, typename RL, int E, int T, int PS>
+__global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
+    using C = typename cvec<R>::type;
+    constexpr int N = RL::n();
+    constexpr int P = N / E;
+    constexpr int RS = N + (N >> PS) + 1;
+    __shared__ C lds[T * RS];
+
+    const int tid = threadIdx.x;
+    const int blk = blockIdx.x;
+    const int cb = blk % a.ncolblk;
+    const int rest = blk / a.ncolblk;
+    const int b1 = rest % a.nb1;
+    const int b2 = rest / a.nb1;
+    const int c0 = cb * T;
+
+    int l, t, l2, t2;
+    if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
+    if (a.store_colfast) { l2 = tid % T; t2 = tid / T; } else { l2 = tid / P; t2 = tid % P; }
+
+    const R sg = a.conj ? R(-1) : R(1);
+    C v[E];
+    {
+        const bool valid = (c0 + l) < a.ncol;
+        const long long base = (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2 + (long long)(c0 + l) * a.in_sc;
+        if (a.in_real) {
+            const R *__restrict__ src = static_cast<const R *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                v[e] = valid ? cmk<C>(src[base + (long long)(t + P * e) * a.in_sa], R(0)) : cmk<C>(R(0), R(0));
+        } else if (a.herm_load) {
+            const C *__restrict__ src = static_cast<const C *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int pos = t + P * e;
+                const bool mirror = pos > N / 2;
+                C x = valid ? src[base + (long long)(mirror ? N - pos : pos) * a.in_sa] : cmk<C>(R(0), R(0));
+                v[e] = cmk<C>(x.x, (mirror ? -sg : sg) * x.y);
+            }
+        } else {
+            const C *__restrict__ src = static_cast<const C *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                C x = valid ? src[base + (long long)(t + P * e) * a.in_sa] : cmk<C>(R(0), R(0));
+                v[e] = cmk<C>(x.x, sg * x.y);
+            }
+        }
+    }
+
+    const C *__restrict__ tw = static_cast<const C *>(a.tw);
+    run_stages<0, RL, E, P, RS, PS>(v, lds, tw, l, t, l2, t2);
+    if constexpr (RL::count == 1) {
+        if (a.load_colfast != a.store_colfast) {
+#pragma unroll
+            for (int e = 0; e < E; e++) lds[l * RS + padj<PS>(t + P * e)] = v[e];
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = lds[l2 * RS + padj<PS>(t2 + P * e)];
+        }
+    }
+
+    if ((c0 + l2) < a.ncol) {
+        const R s = (R)a.scale;
+        const long long base =
+            (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)(c0 + l2) * a.out_sc;
+        if (a.out_real) {
+            R *__restrict__ dst = static_cast<R *>(a.out);
+#pragma unroll
+            for (int e = 0; e < E; e++) dst[base + (long long)(t2 + P * e) * a.out_sa] = v[e].x * s;
+        } else if (a.store_len < N) {
+            C *__restrict__ dst = static_cast<C *>(a.out);
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                if (t2 + P * e < a.store_len)
+                    dst[base + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+        } else {
+            C *__restrict__ dst = static_cast<C *>(a.out);
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                dst[base + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- generic pass
+
+template <int R, typename C>
+__device__ __forceinline__ void generic_bfly(const C *sp, int istride, C *dp, int ostride, const C *w, bool twiddle) {
+    C x[R];
+#pragma unroll
+    for (int p = 0; p < R; p++) x[p] = sp[p * istride];
+    if (twiddle) {
+#pragma unroll
+        for (int p = 1; p < R; p++) x[p] = cmul(x[p], w[p - 1]);
+    }
+    dft<R>::run(x);
+#pragma unroll
+    for (int q = 0; q < R; q++) dp[q * ostride] = x[q];
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) {
+    using C = typename cvec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int N = g.n, T = g.lines;
+    C *buf0 = reinterpret_cast<C *>(smem_raw);
+    C *buf1 = buf0 + (long long)T * N;
+
+    const int tid = threadIdx.x, nth = blockDim.x;
+    const int blk = blockIdx.x;
+    const int cb = blk % a.ncolblk;
+    const int rest = blk / a.ncolblk;
+    const int b1 = rest % a.nb1;
+    const int b2 = rest / a.nb1;
+    const int c0 = cb * T;
+    const R sg = a.conj ? R(-1) : R(1);
+
+    const long long ibase = (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2 + (long long)c0 * a.in_sc;
+    for (int idx = tid; idx < T * N; idx += nth) {
+        int l, j;
+        if (a.load_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
+        C x = cmk<C>(R(0), R(0));
+        if (c0 + l < a.ncol) {
+            const long long off = ibase + (long long)l * a.in_sc + (long long)j * a.in_sa;
+            if (a.in_real) {
+                x.x = static_cast<const R *>(a.in)[off];
+            } else if (a.herm_load && j > N / 2) {
+                x = static_cast<const C *>(a.in)[ibase + (long long)l * a.in_sc + (long long)(N - j) * a.in_sa];
+                x.y *= -sg;
+            } else {
+                x = static_cast<const C *>(a.in)[off];
+                x.y *= sg;
+            }
+        }
+        buf0[l * N + j] = x;
+    }
+    __syncthreads();
+
+    const C *tw = static_cast<const C *>(a.tw);
+    C *src = buf0, *dst = buf1;
+    int Ls = 1;
+    for (int s = 0; s < g.nstages; s++) {
+        const int Rr = g.radix[s];
+        const int nb = N / Rr;
+        for (int idx = tid; idx < T * nb; idx += nth) {
+            const int l = idx / nb, b = idx - (idx / nb) * nb;
+            const int k = b % Ls;
+            const C *sp = src + l * N + b;
+            C *dp = dst + l * N + (b / Ls) * Ls * Rr + k;
+            const C *w = tw + g.twoff[s] + k * (Rr - 1);
+            const bool twd = (k != 0);
+            switch (Rr) {
+                case 2: generic_bfly<2>(sp, nb, dp, Ls, w, twd); break;
+                case 3: generic_bfly<3>(sp, nb, dp, Ls, w, twd); break;
+                case 4: generic_bfly<4>(sp, nb, dp, Ls, w, twd); break;
+                case 5: generic_bfly<5>(sp, nb, dp, Ls, w, twd); break;
+                case 7: generic_bfly<7>(sp, nb, dp, Ls, w, twd); break;
+                case 8: generic_bfly<8>(sp, nb, dp, Ls, w, twd); break;
+                case 11: generic_bfly<11>(sp, nb, dp, Ls, w, twd); break;
+                case 13: generic_bfly<13>(sp, nb, dp, Ls, w, twd); break;
+                case 16: generic_bfly<16>(sp, nb, dp, Ls, w, twd); break;
+                default: {
+                    // any other prime: direct DFT with the w_R^j table
+                    const C *root = tw + g.rootoff[s];
+                    for (int q = 0; q < Rr; q++) {
+                        C acc = sp[0];
+                        for (int p = 1; p < Rr; p++) {
+                            C x = sp[p * nb];
+                            if (twd) x = cmul(x, w[p - 1]);
+                            acc = cadd(acc, cmul(x, root[(p * q) % Rr]));
+                        }
+                        dp[q * Ls] = acc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        C *tmp = src; src = dst; dst = tmp;
+        Ls *= Rr;
+    }
+
+    const R sc = (R)a.scale;
+    const long long obase = (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)c0 * a.out_sc;
+    for (int idx = tid; idx < T * N; idx += nth) {
+        int l, j;
+        if (a.store_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
+        if (c0 + l >= a.ncol || j >= a.store_len) continue;
+        const C y = src[l * N + j];
+        const long long off = obase + (long long)l * a.out_sc + (long long)j * a.out_sa;
+        if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
+        else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
+    }
+}
+
+// ------------------------------------------------------------- permutation
+
+// dst[i so_f + j so_m + k so_s] = src[i si_f + j si_m + k si_s] over a
+// (n_f, n_m, n_s) index box, batched with strides bs_in / bs_out.  The grid
+// tiles the (fi, fo) plane of the source-fastest and destination-fastest axes
+// with 32x32 LDS tiles so that both the read and the write are coalesced.
+template <typename E>
+__global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src, E *__restrict__ dst, permute_args p) {
+    __shared__ E tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    long long blk = blockIdx.x;
+    const long long ta = blk % p.ntile_a; blk /= p.ntile_a;
+    const long long tb = blk % p.ntile_b; blk /= p.ntile_b;
+    const long long c = blk % p.n_c;
+    const long long bt = blk / p.n_c;
+    const long long a0 = ta * 32, b0 = tb * 32;
+    const long long sbase = bt * p.bs_in + c * p.si_c;
+    const long long dbase = bt * p.bs_out + c * p.so_c;
+    // read: a is the source-fastest axis
+    for (int r = ty; r < 32; r += 8) {
+        const long long ia = a0 + tx, ib = b0 + r;
+        if (ia < p.n_a && ib < p.n_b) tile[r][tx] = src[sbase + ia * p.si_a + ib * p.si_b];
+    }
+    __syncthreads();
+    if (p.same_fast) {
+        // both sides are fastest along a: write rows back in the read order
+        for (int r = ty; r < 32; r += 8) {
+            const long long ia = a0 + tx, ib = b0 + r;
+            if (ia < p.n_a && ib < p.n_b) dst[dbase + ia * p.so_a + ib * p.so_b] = tile[r][tx];
+        }
+    } else {
+        // b is the destination-fastest axis
+        for (int r = ty; r < 32; r += 8) {
+            const long long ia = a0 + r, ib = b0 + tx;
+            if (ia < p.n_a && ib < p.n_b) dst[dbase + ia * p.so_a + ib * p.so_b] = tile[tx][r];
+        }
+    }
+}
+
+// ----------------------------------------------------------- fast registry
+
+#define HMI_FAST(TYPE, N, E, T, PS, ...)                                                                  \
+    static hipError_t launch_##TYPE##_##N(const pass_args &a, long long nblocks, hipStream_t s) {          \
+        fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS><<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a); \
+        return hipGetLastError();                                                                          \
+    }
+
+#define HMI_FAST_TABLE(X)                        \
+    X(float, 8, 8, 256, 3, 8)                    \
+    X(float, 16, 16, 128, 4, 16)                 \
+    X(float, 32, 8, 64, 3, 4, 8)                 \
+    X(float, 64, 8, 32, 3, 8, 8)                 \
+    X(float, 128, 16, 32, 4, 16, 8)              \
+    X(float, 256, 16, 16, 4, 16, 16)             \
+    X(float, 512, 8, 16, 3, 8, 8, 8)             \
+    X(float, 1024, 16, 8, 4, 16, 8, 8)           \
+    X(float, 2048, 16, 4, 4, 8, 16, 16)          \
+    X(float, 4096, 16, 4, 4, 16, 16, 16)         \
+    X(float, 12, 12, 128, 4, 4, 3)               \
+    X(float, 20, 20, 64, 4, 4, 5)                \
+    X(float, 24, 12, 64, 4, 3, 4, 2)             \
+    X(float, 27, 9, 64, 4, 3, 3, 3)              \
+    X(float, 28, 28, 64, 4, 4, 7)                \
+    X(float, 40, 20, 64, 4, 2, 4, 5)             \
+    X(float, 45, 15, 64, 4, 3, 3, 5)             \
+    X(float, 48, 12, 32, 4, 4, 4, 3)             \
+    X(float, 56, 28, 64, 4, 2, 4, 7)             \
+    X(float, 63, 21, 64, 4, 3, 3, 7)             \
+    X(float, 80, 20, 32, 4, 4, 4, 5)             \
+    X(float, 96, 12, 16, 4, 4, 4, 2, 3)          \
+    X(float, 112, 28, 32, 4, 4, 4, 7)            \
+    X(double, 8, 8, 128, 3, 8)                   \
+    X(double, 16, 16, 64, 4, 16)                 \
+    X(double, 32, 8, 32, 3, 4, 8)                \
+    X(double, 64, 8, 16, 3, 8, 8)                \
+    X(double, 128, 16, 16, 4, 16, 8)             \
+    X(double, 256, 16, 8, 4, 16, 16)             \
+    X(double, 512, 8, 8, 3, 8, 8, 8)             \
+    X(double, 1024, 16, 4, 4, 16, 8, 8)          \
+    X(double, 2048, 16, 2, 4, 8, 16, 16)         \
+    X(double, 4096, 16, 2, 4, 16, 16, 16)
+
+HMI_FAST_TABLE(HMI_FAST)
+
+#define HMI_FAST_ENTRY(TYPE, N, E, T, PS, ...) \
+    {sizeof(TYPE) == 8, N, E, T, {__VA_ARGS__}, radices<__VA_ARGS__>::count, &launch_##TYPE##_##N},
+
+static const fast_kernel_info kFast[] = {HMI_FAST_TABLE(HMI_FAST_ENTRY)};
+
+const fast_kernel_info *find_fast_kernel(int n, bool dbl) {
+    for (const auto &k : kFast)
+        if (k.n == n && k.dbl == dbl) return &k;
+    return nullptr;
+}
+
+hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,
+                          hipStream_t s) {
+    static bool configured = false;
+    if (!configured) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_generic<float>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_generic<double>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        configured = true;
+    }
+    if (dbl) fft_generic<double><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(a, g);
+    else fft_generic<float><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(a, g);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute(int elem_bytes, const void *src, void *dst, const permute_args &p, hipStream_t s) {
+    const long long nblocks = p.ntile_a * p.ntile_b * p.n_c * p.nbatch;
+    if (nblocks <= 0) return hipSuccess;
+    const dim3 grid((unsigned)nblocks), block(256);
+    switch (elem_bytes) {
+        case 4: permute_kernel<uint32_t><<<grid, block, 0, s>>>((const uint32_t *)src, (uint32_t *)dst, p); break;
+        case 8: permute_kernel<uint2><<<grid, block, 0, s>>>((const uint2 *)src, (uint2 *)dst, p); break;
+        case 16: permute_kernel<uint4><<<grid, block, 0, s>>>((const uint4 *)src, (uint4 *)dst, p); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hmi

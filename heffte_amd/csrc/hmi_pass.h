@@ -1,0 +1,48 @@
+// Host/device description of one FFT pass: a batched 1D c2c transform of
+// length n along one axis of a 3D array, reading one layout and writing another.
+//
+// A pass sees its data as lines (the FFT axis, stride sa) indexed by a column
+// (stride sc), a third axis b1 (stride sb1) and the batch b2 (stride sb2), with
+// independent strides on the input and on the output side.  The layout
+// permutation heFFTe performs with reshape3d_transpose
+// (include/heffte_reshape3d.h:443-487) is therefore folded into the strides of
+// the load or of the store of a pass, and the scale of apply_scale
+// (include/heffte_fft3d.h:662-678) into its store.
+#pragma once
+#include <cstdint>
+
+namespace hmi {
+
+enum : int { kMaxStages = 24 };
+
+struct pass_args {
+    const void *in;
+    void *out;
+    const void *tw;  // twiddle table of the pass (real2 of the pass precision)
+    long long in_sa, in_sc, in_sb1, in_sb2;
+    long long out_sa, out_sc, out_sb1, out_sb2;
+    int ncol;      // number of columns
+    int nb1;       // extent of the third axis
+    int nbatch;    // number of transforms in the batch
+    int ncolblk;   // ceil(ncol / lines per block)
+    int load_colfast;   // 1: consecutive threads walk the column axis on load
+    int store_colfast;  // 1: ... on store
+    int conj;      // 1: backward transform (conjugate on load and on store)
+    int in_real;   // 1: input elements are real scalars (imaginary part 0)
+    int out_real;  // 1: store only the real part
+    int store_len; // positions >= store_len along the FFT axis are not stored (r2c half spectrum)
+    int herm_load; // 1: positions > n/2 are read as conj(x[n - pos]) (c2r Hermitian extension)
+    double scale;  // multiplies every stored value (cast to the pass precision)
+};
+
+// Runtime radix plan used by the generic (any length) pass.
+struct generic_plan {
+    int n;
+    int nstages;
+    int lines;                 // lines per block
+    int radix[kMaxStages];
+    int twoff[kMaxStages];     // offset (complex units) of stage twiddles w_{Ls R}^{p k}
+    int rootoff[kMaxStages];   // offset of w_R^j, j < R (used by radices without a hard-coded butterfly)
+};
+
+}  // namespace hmi

@@ -1,0 +1,480 @@
+// Planner and executor of the single-rank 3D transform (see hmi_plan.h).
+#include "hmi_plan.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <stdexcept>
+
+namespace hmi {
+
+namespace {
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+bool is_permutation(const int o[3]) {
+    bool seen[3] = {false, false, false};
+    for (int i = 0; i < 3; i++) {
+        if (o[i] < 0 || o[i] > 2 || seen[o[i]]) return false;
+        seen[o[i]] = true;
+    }
+    return true;
+}
+
+const int kOrders[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+// Radix decomposition of the generic pass: 8s, then 4, 2, then odd primes.
+std::vector<int> factorize(int n) {
+    std::vector<int> r;
+    int m = n;
+    while (m % 8 == 0) { r.push_back(8); m /= 8; }
+    if (m % 4 == 0) { r.push_back(4); m /= 4; }
+    if (m % 2 == 0) { r.push_back(2); m /= 2; }
+    for (int p = 3; m > 1; p += 2) {
+        while (m % p == 0) { r.push_back(p); m /= p; }
+        if ((long long)p * p > m && m > 1) { r.push_back(m); m = 1; }
+    }
+    return r;
+}
+
+bool has_butterfly(int r) {
+    return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13 || r == 16;
+}
+
+struct candidate_pass {
+    int axis;
+    bool identity;
+};
+
+}  // namespace
+
+layout layout::ordered(const long long n[3], const int order[3]) {
+    layout L;
+    for (int d = 0; d < 3; d++) L.n[d] = n[d];
+    L.stride[order[0]] = 1;
+    L.stride[order[1]] = n[order[0]];
+    L.stride[order[2]] = n[order[0]] * n[order[1]];
+    return L;
+}
+
+int layout::fastest() const {
+    int best = -1;
+    for (int d = 0; d < 3; d++)
+        if (n[d] > 1 && (best < 0 || stride[d] < stride[best])) best = d;
+    return best;
+}
+
+fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
+    : kind_(k), inbox_(inbox), outbox_(outbox), r2c_dir_(r2c_dir) {
+    if (!is_permutation(inbox.order) || !is_permutation(outbox.order))
+        throw std::invalid_argument("box order must be a permutation of (0, 1, 2)");
+    if (inbox.empty() || outbox.empty()) throw std::invalid_argument("empty box: a single rank must own the whole world");
+    if (k == kind::r2c) {
+        if (r2c_dir < 0 || r2c_dir > 2) throw std::invalid_argument("r2c direction must be 0, 1 or 2");
+        // box3d::r2c (include/heffte_geometry.h:96-104)
+        for (int d = 0; d < 3; d++) {
+            long long hi = (d == r2c_dir) ? inbox.low[d] + inbox.size(d) / 2 : inbox.high[d];
+            if (outbox.low[d] != inbox.low[d] || outbox.high[d] != hi)
+                throw std::invalid_argument("outbox must equal inbox.r2c(r2c_direction) on a single rank");
+        }
+    } else {
+        if (!inbox.same_indexes(outbox))
+            throw std::invalid_argument("inbox and outbox must hold the same indexes on a single rank");
+    }
+    if (k == kind::cos || k == kind::sin) throw std::invalid_argument("r2r transforms are not available yet");
+    // workspace: the largest over every variant the plan may run
+    long long ws = 0;
+    for (int dir : {-1, 1})
+        for (int ir = 0; ir < 2; ir++)
+            for (int orl = 0; orl < 2; orl++)
+                for (int ip = 0; ip < 2; ip++) {
+                    if (k == kind::c2c && ir && dir == 1) continue;
+                    if (k == kind::c2c && orl && dir == -1) continue;
+                    if (k == kind::r2c && (ir != (dir == -1) || orl != (dir == 1))) continue;
+                    exec_plan p = build(dir, false, ir, orl, ip);
+                    ws = std::max(ws, p.ws_elems);
+                }
+    ws_elems_ = ws;
+}
+
+fft_plan::~fft_plan() {
+    for (auto &kv : tw_)
+        if (kv.second.dev) (void)hipFree(kv.second.dev);
+}
+
+double fft_plan::scale_factor(int scaling) const {
+    // include/heffte_fft3d.h:577 (1 / N of the real-space world) and :545-547
+    const double n = (double)inbox_.count();
+    double s = 1.0 / n;
+    if (scaling == 0) return 1.0;
+    if (scaling == 2) return std::sqrt(s);
+    return s;
+}
+
+const exec_plan &fft_plan::get_exec_plan(int dir, bool dbl, bool in_real, bool out_real, bool in_place) {
+    const int key = ((dir > 0) << 4) | (dbl << 3) | (in_real << 2) | (out_real << 1) | (int)in_place;
+    std::lock_guard<std::mutex> lock(mtx_);
+    auto it = plans_.find(key);
+    if (it == plans_.end()) it = plans_.emplace(key, build(dir, dbl, in_real, out_real, in_place)).first;
+    return it->second;
+}
+
+exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const {
+    const bool fwd = dir < 0;
+    const box3 &bin = fwd ? inbox_ : outbox_;
+    const box3 &bout = fwd ? outbox_ : inbox_;
+    long long nin[3], nout[3];
+    for (int d = 0; d < 3; d++) { nin[d] = bin.size(d); nout[d] = bout.size(d); }
+    const layout Lin = layout::ordered(nin, bin.order);
+    const layout Lout = layout::ordered(nout, bout.order);
+    const bool r2c = (kind_ == kind::r2c);
+    // full (real-space) extents; the half-spectrum extents differ along r2c_dir
+    long long nfull[3];
+    for (int d = 0; d < 3; d++) nfull[d] = inbox_.size(d);
+
+    // axes to transform
+    std::vector<int> axes;
+    if (r2c && fwd) axes.push_back(r2c_dir_);
+    for (int d = 0; d < 3; d++)
+        if (nfull[d] > 1 && !(r2c && d == r2c_dir_)) axes.push_back(d);
+    if (r2c && !fwd) axes.push_back(r2c_dir_);
+    if (axes.empty()) axes.push_back(0);
+
+    // enumerate axis orders x intermediate layouts x intermediate buffers
+    std::vector<int> perm = axes;
+    std::sort(perm.begin(), perm.end());
+    double best_cost = std::numeric_limits<double>::infinity();
+    exec_plan best;
+
+    auto try_plan = [&](const std::vector<candidate_pass> &seq) {
+        const int k = (int)seq.size();
+        int nlay = 1, nbuf = 1;
+        for (int i = 0; i + 1 < k; i++) { nlay *= 6; nbuf *= 2; }
+        for (int li = 0; li < nlay; li++) {
+            for (int bi = 0; bi < nbuf; bi++) {
+                exec_plan ep;
+                double cost = 0.0;
+                bool ok = true;
+                bool uses_ws = false;
+                long long dims[3] = {nin[0], nin[1], nin[2]};
+                layout cur = Lin;
+                buf cur_buf = buf::in;
+                int lcode = li, bcode = bi;
+                for (int i = 0; i < k && ok; i++) {
+                    const candidate_pass &cp = seq[i];
+                    pass p;
+                    p.axis = cp.axis;
+                    p.last = (i + 1 == k);
+                    long long odims[3] = {dims[0], dims[1], dims[2]};
+                    if (r2c && cp.axis == r2c_dir_ && !cp.identity) {
+                        if (fwd) odims[r2c_dir_] = nfull[r2c_dir_] / 2 + 1;
+                        else odims[r2c_dir_] = nfull[r2c_dir_];
+                    }
+                    p.n = cp.identity ? (int)dims[cp.axis] : (int)nfull[cp.axis];
+                    if (r2c && cp.axis == r2c_dir_ && !cp.identity) {
+                        if (fwd) p.store_len = (int)odims[r2c_dir_];
+                        else p.herm_load = 1;
+                    }
+                    if (!p.store_len) p.store_len = p.n;
+                    p.identity = cp.identity;
+                    p.lin = cur;
+                    p.src = cur_buf;
+                    if (p.last) {
+                        p.lout = Lout;
+                        p.dst = buf::out;
+                    } else {
+                        p.lout = layout::ordered(odims, kOrders[lcode % 6]);
+                        lcode /= 6;
+                        p.dst = (bcode % 2) ? buf::ws : buf::out;
+                        bcode /= 2;
+                    }
+                    p.in_real = (i == 0 && in_real) ? 1 : 0;
+                    p.out_real = (p.last && out_real) ? 1 : 0;
+                    // an intermediate in OUT must fit the output buffer as complex data
+                    if (!p.last && p.dst == buf::out) {
+                        const long long cnt = odims[0] * odims[1] * odims[2];
+                        if (out_real || cnt > bout.count()) { ok = false; break; }
+                    }
+                    if (p.dst == buf::ws) uses_ws = true;
+                    // coalescing: pick the column axis
+                    const int fin = p.lin.fastest(), fout = p.lout.fastest();
+                    p.load_colfast = (fin >= 0 && fin != p.axis);
+                    p.store_colfast = (fout >= 0 && fout != p.axis);
+                    if (p.load_colfast && p.store_colfast) {
+                        if (fin != fout) { ok = false; break; }
+                        p.col = fin;
+                    } else if (p.load_colfast) {
+                        p.col = fin;
+                    } else if (p.store_colfast) {
+                        p.col = fout;
+                    } else {
+                        int c = -1;
+                        for (int d = 0; d < 3; d++) {
+                            if (d == p.axis) continue;
+                            if (c < 0 || (dims[d] > 1 && (dims[c] <= 1 || p.lin.stride[d] < p.lin.stride[c]))) c = d;
+                        }
+                        p.col = c;
+                    }
+                    p.b1 = 3 - p.axis - p.col;
+                    // in-place safety: same buffer => identical layout and element type
+                    const bool same_mem = (p.src == p.dst) ||
+                                          (in_place && ((p.src == buf::in && p.dst == buf::out) ||
+                                                        (p.src == buf::out && p.dst == buf::in)));
+                    if (same_mem) {
+                        bool preserve = !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
+                        for (int d = 0; d < 3 && preserve; d++)
+                            if (dims[d] != odims[d] || (dims[d] > 1 && p.lin.stride[d] != p.lout.stride[d]))
+                                preserve = false;
+                        if (!preserve) { ok = false; break; }
+                    }
+                    cost += 1.0;
+                    if (p.load_colfast && p.store_colfast) cost += 0.15;
+                    else if (p.load_colfast || p.store_colfast) cost += 0.05;
+                    if (cp.identity) cost += 0.5;
+                    ep.passes.push_back(p);
+                    cur = p.lout;
+                    cur_buf = p.dst;
+                    for (int d = 0; d < 3; d++) dims[d] = odims[d];
+                }
+                if (!ok) continue;
+                if (uses_ws) cost += 0.01;
+                if (cost < best_cost) {
+                    long long ws = 0;
+                    for (const pass &p : ep.passes)
+                        if (p.dst == buf::ws) ws = std::max(ws, p.lout.n[0] * p.lout.n[1] * p.lout.n[2]);
+                    ep.ws_elems = ws;
+                    best_cost = cost;
+                    best = ep;
+                }
+            }
+        }
+    };
+
+    do {
+        if (r2c && fwd && perm[0] != r2c_dir_) continue;
+        if (r2c && !fwd && perm.back() != r2c_dir_) continue;
+        std::vector<candidate_pass> seq;
+        for (int a : perm) seq.push_back({a, false});
+        try_plan(seq);
+        if (!std::isfinite(best_cost)) {
+            // no valid in-place schedule with one pass per axis: append a copy pass
+            for (int d = 0; d < 3; d++) {
+                std::vector<candidate_pass> s2 = seq;
+                s2.push_back({d, true});
+                try_plan(s2);
+            }
+        }
+    } while (std::next_permutation(perm.begin(), perm.end()));
+    if (!std::isfinite(best_cost)) throw std::runtime_error("no valid pass schedule for these boxes");
+
+    for (pass &p : best.passes) configure(p, dbl);
+    return best;
+}
+
+void fft_plan::configure(pass &p, bool dbl) const {
+    const size_t elem = dbl ? 16 : 8;
+    p.fast = p.identity ? nullptr : find_fast_kernel(p.n, dbl);
+    if (p.fast) {
+        p.lines = p.fast->T;
+        return;
+    }
+    generic_plan &g = p.gen;
+    g = generic_plan{};
+    g.n = p.n;
+    std::vector<int> rad = p.identity ? std::vector<int>{} : factorize(p.n);
+    if ((int)rad.size() > kMaxStages) throw std::runtime_error("too many radix stages");
+    g.nstages = (int)rad.size();
+    int off = 0, ls = 1;
+    for (int s = 0; s < g.nstages; s++) {
+        g.radix[s] = rad[s];
+        g.twoff[s] = off;
+        off += ls * (rad[s] - 1);
+        ls *= rad[s];
+    }
+    for (int s = 0; s < g.nstages; s++) {
+        g.rootoff[s] = off;
+        if (!has_butterfly(rad[s])) off += rad[s];
+    }
+    const size_t per_line = 2 * (size_t)p.n * elem;
+    if (per_line > (size_t)kGenericMaxLds)
+        throw std::runtime_error("transform length " + std::to_string(p.n) + " exceeds the single-pass LDS capacity");
+    int lines = (int)std::min<size_t>(64, std::max<size_t>(1, (64 * 1024) / per_line));
+    // no more lines per block than columns
+    const long long ncol = p.lin.n[p.col] > 0 ? std::max(p.lin.n[p.col], p.lout.n[p.col]) : 1;
+    while (lines > 1 && lines / 2 >= ncol) lines /= 2;
+    g.lines = lines;
+    p.lines = lines;
+}
+
+namespace {
+
+// exp(-2 pi i e / m) rounded from long double
+template <typename R> void root(long long e, long long m, R &re, R &im) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    e %= m;
+    const long double a = 2.0L * pi * (long double)e / (long double)m;
+    re = (R)cosl(a);
+    im = (R)(-sinl(a));
+}
+
+template <typename R>
+std::vector<R> make_table(const int *radix, int nstages, bool with_roots) {
+    std::vector<R> t;
+    long long ls = 1;
+    for (int s = 0; s < nstages; s++) {
+        const int r = radix[s];
+        for (long long k = 0; k < ls; k++)
+            for (int p = 1; p < r; p++) {
+                R re, im;
+                root<R>((long long)p * k, ls * r, re, im);
+                t.push_back(re);
+                t.push_back(im);
+            }
+        ls *= r;
+    }
+    if (with_roots)
+        for (int s = 0; s < nstages; s++) {
+            if (has_butterfly(radix[s])) continue;
+            for (int j = 0; j < radix[s]; j++) {
+                R re, im;
+                root<R>(j, radix[s], re, im);
+                t.push_back(re);
+                t.push_back(im);
+            }
+        }
+    if (t.empty()) { t.push_back(R(1)); t.push_back(R(0)); }
+    return t;
+}
+
+}  // namespace
+
+const twiddles &fft_plan::get_twiddles(int n, bool dbl, const pass &p) {
+    const long long key = ((long long)n << 3) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0);
+    std::lock_guard<std::mutex> lock(mtx_);
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second;
+    const int *radix = p.fast ? p.fast->radix : p.gen.radix;
+    const int ns = p.fast ? p.fast->nstages : p.gen.nstages;
+    twiddles t;
+    if (dbl) {
+        std::vector<double> v = make_table<double>(radix, ns, !p.fast);
+        t.bytes = v.size() * sizeof(double);
+        check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
+        check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+    } else {
+        std::vector<float> v = make_table<float>(radix, ns, !p.fast);
+        t.bytes = v.size() * sizeof(float);
+        check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
+        check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+    }
+    return tw_.emplace(key, t).first->second;
+}
+
+void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
+                       int scaling) {
+    if (batch < 1) throw std::invalid_argument("batch size must be positive");
+    if (!in || !out) throw std::invalid_argument("null input or output array");
+    const bool in_place = (in == out);
+    const exec_plan &ep = get_exec_plan(dir, dbl, in_real, out_real, in_place);
+    void *tmp = nullptr;
+    if (ep.ws_elems > 0 && !ws) {
+        // fft3d::forward(in, out) allocates the workspace per call (include/heffte_fft3d.h:353-360)
+        check_hip(hipMallocAsync(&tmp, (size_t)ep.ws_elems * batch * (dbl ? 16 : 8), stream_), "hipMallocAsync(workspace)");
+        ws = tmp;
+    }
+    const double scale = scale_factor(scaling);
+    // capacity of each buffer (elements of its own type) under the fft3d contract
+    const bool fwd = dir < 0;
+    const long long cap_in = batch * (fwd ? inbox_.count() : outbox_.count());
+    const long long cap_out = batch * (fwd ? outbox_.count() : inbox_.count());
+    const long long cap_ws = batch * ep.ws_elems;
+    auto capacity = [&](buf b) { return b == buf::in ? cap_in : (b == buf::out ? cap_out : cap_ws); };
+    auto extent = [&](const layout &L) {
+        long long e = 0;
+        for (int d = 0; d < 3; d++) e += (L.n[d] - 1) * L.stride[d];
+        return e + (long long)(batch - 1) * (L.n[0] * L.n[1] * L.n[2]) + 1;
+    };
+    const bool debug = std::getenv("HMI_DEBUG_PLAN") != nullptr;
+    for (const pass &p : ep.passes) {
+        // every element a pass touches lies inside the buffer it names
+        if (extent(p.lin) > capacity(p.src) || extent(p.lout) > capacity(p.dst))
+            throw std::runtime_error("internal error: pass exceeds its buffer");
+        if (debug)
+            std::fprintf(stderr,
+                         "[hmi] pass axis=%d n=%d src=%d dst=%d col=%d b1=%d lcf=%d scf=%d real=%d/%d herm=%d "
+                         "store_len=%d fast=%d lines=%d in_stride=(%lld,%lld,%lld) out_stride=(%lld,%lld,%lld)\n",
+                         p.axis, p.n, (int)p.src, (int)p.dst, p.col, p.b1, p.load_colfast, p.store_colfast,
+                         p.in_real, p.out_real, p.herm_load, p.store_len, p.fast ? 1 : 0, p.lines, p.lin.stride[0],
+                         p.lin.stride[1], p.lin.stride[2], p.lout.stride[0], p.lout.stride[1], p.lout.stride[2]);
+        pass_args a{};
+        auto bufptr = [&](buf b) -> void * {
+            return b == buf::in ? const_cast<void *>(in) : (b == buf::out ? out : ws);
+        };
+        a.in = bufptr(p.src);
+        a.out = bufptr(p.dst);
+        const twiddles &tw = get_twiddles(p.n, dbl, p);
+        a.tw = tw.dev;
+        a.in_sa = p.lin.stride[p.axis];
+        a.in_sc = p.lin.stride[p.col];
+        a.in_sb1 = p.lin.stride[p.b1];
+        a.in_sb2 = p.lin.n[0] * p.lin.n[1] * p.lin.n[2];
+        a.out_sa = p.lout.stride[p.axis];
+        a.out_sc = p.lout.stride[p.col];
+        a.out_sb1 = p.lout.stride[p.b1];
+        a.out_sb2 = p.lout.n[0] * p.lout.n[1] * p.lout.n[2];
+        a.ncol = (int)p.lout.n[p.col];
+        a.nb1 = (int)p.lout.n[p.b1];
+        a.nbatch = batch;
+        a.ncolblk = (a.ncol + p.lines - 1) / p.lines;
+        a.load_colfast = p.load_colfast;
+        a.store_colfast = p.store_colfast;
+        a.conj = dir > 0 ? 1 : 0;
+        a.in_real = p.in_real;
+        a.out_real = p.out_real;
+        a.store_len = p.store_len;
+        a.herm_load = p.herm_load;
+        a.scale = p.last ? scale : 1.0;
+        const long long nblocks = (long long)a.ncolblk * a.nb1 * batch;
+        if (nblocks > 0x7fffffffLL) throw std::runtime_error("grid too large");
+        if (p.fast) {
+            check_hip(p.fast->launch(a, nblocks, stream_), "fast FFT pass");
+        } else {
+            const size_t lds = 2 * (size_t)p.gen.lines * p.n * (dbl ? 16 : 8);
+            check_hip(launch_generic(dbl, a, p.gen, nblocks, lds, stream_), "generic FFT pass");
+        }
+    }
+    if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
+}
+
+void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,
+             hipStream_t s) {
+    if (!is_permutation(inbox.order) || !is_permutation(out_order))
+        throw std::invalid_argument("box order must be a permutation of (0, 1, 2)");
+    if (elem_bytes != 4 && elem_bytes != 8 && elem_bytes != 16)
+        throw std::invalid_argument("element size must be 4, 8 or 16 bytes");
+    if (batch < 1) throw std::invalid_argument("batch size must be positive");
+    if (inbox.empty()) return;
+    if (!in || !out) throw std::invalid_argument("null input or output array");
+    if (in == out) throw std::invalid_argument("the permutation kernel is out-of-place");
+    long long n[3];
+    for (int d = 0; d < 3; d++) n[d] = inbox.size(d);
+    const layout Li = layout::ordered(n, inbox.order), Lo = layout::ordered(n, out_order);
+    int a = inbox.order[0], b = out_order[0];
+    permute_args p{};
+    p.same_fast = (a == b);
+    if (a == b) b = inbox.order[1];
+    const int c = 3 - a - b;
+    p.n_a = n[a]; p.n_b = n[b]; p.n_c = n[c]; p.nbatch = batch;
+    p.si_a = Li.stride[a]; p.si_b = Li.stride[b]; p.si_c = Li.stride[c]; p.bs_in = inbox.count();
+    p.so_a = Lo.stride[a]; p.so_b = Lo.stride[b]; p.so_c = Lo.stride[c]; p.bs_out = inbox.count();
+    p.ntile_a = (n[a] + 31) / 32;
+    p.ntile_b = (n[b] + 31) / 32;
+    if (p.ntile_a * p.ntile_b * p.n_c * p.nbatch > 0x7fffffffLL) throw std::runtime_error("grid too large");
+    check_hip(launch_permute(elem_bytes, in, out, p, s), "permute kernel");
+}
+
+}  // namespace hmi

@@ -1,0 +1,118 @@
+// Host-side plan of a single-rank 3D transform on one MI355X.
+//
+// A plan is what heFFTe builds in fft3d::setup (include/heffte_fft3d.h:599-651)
+// for a communicator of size one: the reshapes and 1D executors of
+// compute_transform (src/heffte_compute_transform.cpp:15-258), re-expressed as
+// a short list of fused passes.  Each pass transforms one axis and writes the
+// layout the next pass reads best; the reshapes are folded into the load or
+// the store of a pass and the scale into the store of the last pass.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hmi_kernels.h"
+
+namespace hmi {
+
+enum class kind { c2c = 0, r2c = 1, cos = 2, sin = 3 };
+enum class buf { in = 0, out = 1, ws = 2 };
+
+struct box3 {
+    long long low[3], high[3];
+    int order[3];
+    long long size(int d) const { return high[d] - low[d] + 1; }
+    long long count() const {
+        for (int d = 0; d < 3; d++)
+            if (size(d) <= 0) return 0;
+        return size(0) * size(1) * size(2);
+    }
+    bool empty() const { return count() == 0; }
+    bool same_indexes(const box3 &o) const {
+        for (int d = 0; d < 3; d++)
+            if (low[d] != o.low[d] || high[d] != o.high[d]) return false;
+        return true;
+    }
+};
+
+// Memory layout of a 3D array: element (i0, i1, i2) at sum_d i_d * stride[d].
+struct layout {
+    long long n[3];
+    long long stride[3];
+    static layout ordered(const long long n[3], const int order[3]);
+    int fastest() const;  // axis with the smallest stride among axes with n > 1 (-1 if none)
+};
+
+struct pass {
+    int axis = 0, n = 1;
+    buf src = buf::in, dst = buf::out;
+    layout lin{}, lout{};
+    int col = 0, b1 = 0;  // column axis and third axis
+    int load_colfast = 0, store_colfast = 0;
+    int in_real = 0, out_real = 0, herm_load = 0, store_len = 0;
+    bool last = false;
+    bool identity = false;  // copy pass (no FFT stages), used when no in-place schedule exists
+    // launch configuration
+    const fast_kernel_info *fast = nullptr;
+    generic_plan gen{};
+    int lines = 1;  // lines (columns) per block
+};
+
+struct exec_plan {
+    std::vector<pass> passes;
+    long long ws_elems = 0;  // workspace needed per transform, in complex elements
+};
+
+// Device twiddle tables for one length and precision.
+struct twiddles {
+    void *dev = nullptr;
+    size_t bytes = 0;
+};
+
+class fft_plan {
+  public:
+    fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir);
+    ~fft_plan();
+
+    kind get_kind() const { return kind_; }
+    long long size_inbox() const { return inbox_.count(); }
+    long long size_outbox() const { return outbox_.count(); }
+    long long size_workspace() const { return ws_elems_; }
+    int r2c_dir() const { return r2c_dir_; }
+    double scale_factor(int scaling) const;
+    void set_stream(hipStream_t s) { stream_ = s; }
+    hipStream_t stream() const { return stream_; }
+
+    // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
+    // overloads of fft3d (real input on forward, real output on backward).
+    void execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
+                 int scaling);
+
+    // Plan (without running it): exposed for tests of the planner.
+    const exec_plan &get_exec_plan(int dir, bool dbl, bool in_real, bool out_real, bool in_place);
+
+  private:
+    exec_plan build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const;
+    void configure(pass &p, bool dbl) const;
+    const twiddles &get_twiddles(int n, bool dbl, const pass &p);
+
+    kind kind_;
+    box3 inbox_, outbox_;
+    int r2c_dir_;
+    long long ws_elems_ = 0;
+    hipStream_t stream_ = nullptr;
+    std::mutex mtx_;
+    std::map<int, exec_plan> plans_;
+    std::map<long long, twiddles> tw_;
+};
+
+// Permutation of a box between two orders (reshape3d_transpose semantics).
+void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,
+             hipStream_t s);
+
+}  // namespace hmi

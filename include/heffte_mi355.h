@@ -1,0 +1,151 @@
+/*
+ * heffte_mi355.h -- C ABI of the MI355X (gfx950) single-rank 3D FFT backend.
+ *
+ * The entry points mirror heFFTe's C binding (include/heffte_c.h) for a
+ * communicator of size one: plan creation from boxes with an index order,
+ * size queries, and forward/backward calls with a caller-provided workspace and
+ * a scale mode.  Differences, all forced by the boundary being a plain C ABI:
+ *   - every call returns a status code (no C++ exception crosses the ABI);
+ *   - there is no MPI_Comm argument: the plan covers one rank that owns the
+ *     whole world, so inbox and outbox hold the same indexes;
+ *   - box indexes are long long (heFFTe's box3d<long long> covers box3d<int>);
+ *   - a batch size argument carries fft3d::forward(batch_size, ...)
+ *     (include/heffte_fft3d.h:392-402) through the ABI;
+ *   - the HIP stream is bound to the plan (fft3d(gpu_stream, ...),
+ *     include/heffte_fft3d.h:287-290) by heffte_mi355_set_stream.
+ *
+ * All pointers passed to the forward/backward calls are device pointers.
+ * Calls are asynchronous on the plan's stream, as heFFTe's GPU backends are.
+ */
+#ifndef HEFFTE_MI355_H
+#define HEFFTE_MI355_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Backend id a maintainer would add next to Heffte_BACKEND_ROCFFT
+ * (include/heffte_c_defines.h:82). */
+#define Heffte_BACKEND_MI355 12
+
+/* Status codes; 0-3 keep the meaning of include/heffte_c.h:49-50,74. */
+#define HMI_SUCCESS 0
+#define HMI_ERROR_INVALID_BACKEND 1
+#define HMI_ERROR_INVALID_ARGUMENT 2
+#define HMI_ERROR_CORRUPTED_PLAN 3
+#define HMI_ERROR_DEVICE 4
+
+/* Scale modes, same values as Heffte_SCALE_* (include/heffte_c_defines.h:152-162). */
+#define HMI_SCALE_NONE 0
+#define HMI_SCALE_FULL 1
+#define HMI_SCALE_SYMMETRIC 2
+
+/* Transform kinds. */
+#define HMI_KIND_C2C 0 /* heffte::fft3d                                   */
+#define HMI_KIND_R2C 1 /* heffte::fft3d_r2c                               */
+#define HMI_KIND_COS 2 /* heffte::rtransform<backend::*_cos> (DCT-II/III) */
+#define HMI_KIND_SIN 3 /* heffte::rtransform<backend::*_sin> (DST-II/III) */
+
+/* Same layout as heffte_plan_options (include/heffte_c_defines.h:113-122).
+ * On a single rank the options do not change any result; they are accepted
+ * so that callers can pass their existing option structs. */
+typedef struct {
+    int use_reorder;
+    int algorithm;
+    int use_pencils;
+    int use_gpu_aware;
+} heffte_mi355_plan_options;
+
+typedef struct heffte_mi355_plan_s *heffte_mi355_plan;
+
+/* Replaces heffte_set_default_options (include/heffte_c.h:25). */
+int heffte_mi355_set_default_options(heffte_mi355_plan_options *options);
+
+/* Replaces heffte_plan_create (include/heffte_c.h:52-54).  inbox/outbox
+ * orders may be NULL for (0, 1, 2).  options may be NULL. */
+int heffte_mi355_plan_create(long long const inbox_low[3], long long const inbox_high[3], int const *inbox_order,
+                             long long const outbox_low[3], long long const outbox_high[3], int const *outbox_order,
+                             heffte_mi355_plan_options const *options, heffte_mi355_plan *plan);
+
+/* Replaces heffte_plan_create_r2c (include/heffte_c.h:63-65). */
+int heffte_mi355_plan_create_r2c(long long const inbox_low[3], long long const inbox_high[3], int const *inbox_order,
+                                 long long const outbox_low[3], long long const outbox_high[3],
+                                 int const *outbox_order, int r2c_direction,
+                                 heffte_mi355_plan_options const *options, heffte_mi355_plan *plan);
+
+/* Replaces the construction of heffte::rtransform<tag> (include/heffte_fft3d.h:748-749;
+ * the reference C binding has no r2r entry).  kind is HMI_KIND_COS or HMI_KIND_SIN. */
+int heffte_mi355_plan_create_r2r(int kind, long long const inbox_low[3], long long const inbox_high[3],
+                                 int const *inbox_order, long long const outbox_low[3],
+                                 long long const outbox_high[3], int const *outbox_order,
+                                 heffte_mi355_plan_options const *options, heffte_mi355_plan *plan);
+
+/* Replaces heffte_plan_destroy (include/heffte_c.h:75). */
+int heffte_mi355_plan_destroy(heffte_mi355_plan plan);
+
+/* Replace heffte_size_inbox / _outbox / _workspace (include/heffte_c.h:81-91).
+ * The workspace is counted in elements of the transform's complex type (real
+ * type for r2r), per transform: a batch needs batch * size_workspace. */
+long long heffte_mi355_size_inbox(heffte_mi355_plan plan);
+long long heffte_mi355_size_outbox(heffte_mi355_plan plan);
+long long heffte_mi355_size_workspace(heffte_mi355_plan plan);
+
+/* Replaces heffte_is_r2c (include/heffte_c.h:102); returns the HMI_KIND_* of the plan, -1 on error. */
+int heffte_mi355_get_kind(heffte_mi355_plan plan);
+
+/* Binds a hipStream_t (NULL = default stream); replaces the stream argument of
+ * the fft3d constructor (include/heffte_fft3d.h:287-290). */
+int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream);
+
+/* Forward transforms; replace heffte_forward_{s2c,c2c,d2z,z2z}_buffered
+ * (include/heffte_c.h:164-179) with a batch size.  workspace may be NULL, in
+ * which case it is allocated for the call (fft3d::forward(in, out),
+ * include/heffte_fft3d.h:353-360).  in == out is allowed for c2c/z2z. */
+int heffte_mi355_forward_c2c(heffte_mi355_plan plan, int batch, void const *input, void *output, void *workspace,
+                             int scale);
+int heffte_mi355_forward_z2z(heffte_mi355_plan plan, int batch, void const *input, void *output, void *workspace,
+                             int scale);
+int heffte_mi355_forward_s2c(heffte_mi355_plan plan, int batch, float const *input, void *output, void *workspace,
+                             int scale);
+int heffte_mi355_forward_d2z(heffte_mi355_plan plan, int batch, double const *input, void *output, void *workspace,
+                             int scale);
+
+/* Backward transforms; replace heffte_backward_{c2s,c2c,z2d,z2z}_buffered
+ * (include/heffte_c.h:241-256). */
+int heffte_mi355_backward_c2c(heffte_mi355_plan plan, int batch, void const *input, void *output, void *workspace,
+                              int scale);
+int heffte_mi355_backward_z2z(heffte_mi355_plan plan, int batch, void const *input, void *output, void *workspace,
+                              int scale);
+int heffte_mi355_backward_c2s(heffte_mi355_plan plan, int batch, void const *input, float *output, void *workspace,
+                              int scale);
+int heffte_mi355_backward_z2d(heffte_mi355_plan plan, int batch, void const *input, double *output, void *workspace,
+                              int scale);
+
+/* Real-to-real transforms of an HMI_KIND_COS/SIN plan; replace
+ * rtransform::forward/backward(float/double data, workspace)
+ * (include/heffte_fft3d.h:374-384,480-490 with a real scalar type). */
+int heffte_mi355_forward_s2s(heffte_mi355_plan plan, int batch, float const *input, float *output, void *workspace,
+                             int scale);
+int heffte_mi355_forward_d2d(heffte_mi355_plan plan, int batch, double const *input, double *output,
+                             void *workspace, int scale);
+int heffte_mi355_backward_s2s(heffte_mi355_plan plan, int batch, float const *input, float *output,
+                              void *workspace, int scale);
+int heffte_mi355_backward_d2d(heffte_mi355_plan plan, int batch, double const *input, double *output,
+                              void *workspace, int scale);
+
+/* In-node reshape: moves a box stored in in_order into out_order (same
+ * indexes), element by element and bit-exact.  Replaces
+ * reshape3d_transpose::apply (include/heffte_reshape3d.h:465-484) and the
+ * transpose_unpacker kernel (src/heffte_backend_rocm.hip.cpp:91-134,407-444).
+ * elem_bytes is 4, 8 or 16; input and output must not overlap. */
+int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const high[3], int const in_order[3],
+                         int const out_order[3], int batch, void const *input, void *output, void *hip_stream);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+char const *heffte_mi355_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEFFTE_MI355_H */

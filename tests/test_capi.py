@@ -1,0 +1,111 @@
+"""CPU checks of the C ABI (no kernel launches): the library loads, exports every
+entry point include/heffte_mi355.h declares, and the host-side plan logic
+accepts and rejects boxes with heFFTe's status codes."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "heffte_mi355.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(heffte_mi355_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(hmi):
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(hmi.lib, n), f"{n} is declared in include/heffte_mi355.h but not exported"
+    assert sorted(hmi.C_ABI_SYMBOLS) == names
+
+
+def test_library_has_no_cpu_fallback_symbols(hmi):
+    """The product library links no FFT library: the transforms are its own HIP kernels."""
+    out = os.popen(f"ldd {hmi.LIBRARY_PATH}").read()
+    for banned in ("rocfft", "hipfft", "fftw", "torch", "liboracle"):
+        assert banned not in out, out
+
+
+def LL(v):
+    return (ctypes.c_longlong * 3)(*v)
+
+
+def I3(v):
+    return (ctypes.c_int * 3)(*v)
+
+
+def create(hmi, lo, hi, io, oo):
+    h = ctypes.c_void_p()
+    rc = hmi.lib.heffte_mi355_plan_create(LL(lo), LL(hi), I3(io), LL(lo), LL(hi), I3(oo), None, ctypes.byref(h))
+    return rc, h
+
+
+def test_plan_create_and_sizes(hmi):
+    rc, h = create(hmi, (0, 0, 0), (63, 63, 63), (0, 1, 2), (2, 1, 0))
+    assert rc == 0, hmi.last_error()
+    assert hmi.lib.heffte_mi355_size_inbox(h) == 64 ** 3
+    assert hmi.lib.heffte_mi355_size_outbox(h) == 64 ** 3
+    ws = hmi.lib.heffte_mi355_size_workspace(h)
+    # one full intermediate array at most (heFFTe's own would be 3 N for this plan)
+    assert 0 <= ws <= 64 ** 3
+    assert hmi.lib.heffte_mi355_get_kind(h) == 0
+    assert hmi.lib.heffte_mi355_plan_destroy(h) == 0
+
+
+def test_plan_rejects_bad_boxes(hmi):
+    # inbox and outbox must hold the same indexes on one rank
+    h = ctypes.c_void_p()
+    rc = hmi.lib.heffte_mi355_plan_create(LL((0, 0, 0)), LL((7, 7, 7)), I3((0, 1, 2)), LL((0, 0, 0)),
+                                          LL((7, 7, 6)), I3((0, 1, 2)), None, ctypes.byref(h))
+    assert rc == 2 and "same indexes" in hmi.last_error()
+    # order must be a permutation
+    rc, h = create(hmi, (0, 0, 0), (7, 7, 7), (0, 0, 2), (0, 1, 2))
+    assert rc == 2
+    # empty box
+    rc, h = create(hmi, (0, 0, 0), (7, -1, 7), (0, 1, 2), (0, 1, 2))
+    assert rc == 2
+
+
+def test_r2c_plan_boxes(hmi):
+    inbox = hmi.box3d([0, 0, 0], [15, 9, 7])
+    p = hmi.fft3d_r2c(hmi.backend.mi355, inbox, inbox.r2c(1), 1)
+    assert p.size_inbox() == 16 * 10 * 8 and p.size_outbox() == 16 * 6 * 8
+    with pytest.raises(hmi.heffte_input_error):
+        hmi.fft3d_r2c(hmi.backend.mi355, inbox, inbox.r2c(0), 1)
+    with pytest.raises(hmi.heffte_input_error):
+        hmi.fft3d_r2c(hmi.backend.mi355, inbox, inbox.r2c(0), 3)
+
+
+def test_invalid_plan_and_arguments(hmi):
+    assert hmi.lib.heffte_mi355_plan_destroy(None) == 3
+    assert hmi.lib.heffte_mi355_size_inbox(None) == -1
+    rc, h = create(hmi, (0, 0, 0), (3, 3, 3), (0, 1, 2), (0, 1, 2))
+    # scale outside {0, 1, 2}
+    assert hmi.lib.heffte_mi355_forward_c2c(h, 1, ctypes.c_void_p(16), ctypes.c_void_p(16), None, 7) == 2
+    # null arrays are rejected before any launch
+    assert hmi.lib.heffte_mi355_forward_c2c(h, 1, None, None, None, 0) == 2
+    # r2r call on a complex plan
+    assert hmi.lib.heffte_mi355_forward_d2d(h, 1, ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0) == 2
+    assert hmi.lib.heffte_mi355_plan_destroy(h) == 0
+
+
+def test_communicator_must_have_one_rank(hmi):
+    b = hmi.box3d([0, 0, 0], [3, 3, 3])
+    with pytest.raises(hmi.heffte_input_error):
+        hmi.fft3d(hmi.backend.mi355, b, b, comm=2)
+    hmi.fft3d(hmi.backend.mi355, b, b, comm=1)
+
+
+def test_default_options(hmi):
+    class opts(ctypes.Structure):
+        _fields_ = [("use_reorder", ctypes.c_int), ("algorithm", ctypes.c_int), ("use_pencils", ctypes.c_int),
+                    ("use_gpu_aware", ctypes.c_int)]
+    o = opts()
+    assert hmi.lib.heffte_mi355_set_default_options(ctypes.byref(o)) == 0
+    assert o.use_reorder == 0 and o.use_pencils == 1

@@ -1,0 +1,163 @@
+"""GPU parity of the complex-to-complex 3D transform (fft3d) against the FP64 oracle.
+
+Bounds: relative L2 <= 1e-13 for complex<double>, <= 5e-6 for complex<float>
+(forward and round trip).  Cases follow heFFTe's own tests: the np=1 sizes of
+test/test_fft3d_np1.cpp:17-21 (6x7x8, 7x8x6), the batch case of
+test/test_fft3d.h:505-572 (8x9x10, batch 5), the cross-backend size
+43x76x24 of test/test_units_nompi.cpp:673-809, every in/out order pair, in-place
+and out-of-place calls, caller and internal workspace, and the real-input /
+real-output overloads of test_fft3d_arrays (test/test_fft3d.h:397-503).
+"""
+import numpy as np
+import pytest
+
+import fftcase as fc
+
+pytestmark = pytest.mark.gpu
+
+CDT = {"complex64": np.complex64, "complex128": np.complex128}
+
+
+def run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=0, batch=1, in_place=False, use_ws=True,
+            scaling=0, direction="forward"):
+    import torch
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], inorder)
+    outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], outorder)
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    xs = [fc.uniform_complex(n, seed + b) for b in range(batch)]
+    x = np.concatenate(xs)  # stored in the order of the source box
+    tx = fc.to_gpu(x, CDT[dtype])
+    ty = tx if in_place else torch.empty(batch * n, dtype=tx.dtype, device=cuda)
+    ws = torch.empty(max(1, batch * plan.size_workspace()), dtype=tx.dtype, device=cuda) if use_ws else None
+    if direction == "forward":
+        plan.forward_buffered(tx, ty, ws, scaling, batch=batch) if use_ws else plan.forward(tx, ty, scaling, batch)
+    else:
+        plan.backward_buffered(tx, ty, ws, scaling, batch=batch) if use_ws else plan.backward(tx, ty, scaling, batch)
+    y = fc.to_host(ty)
+    return x, y, plan
+
+
+def oracle_c2c(oracle, size, inorder, outorder, x, sign, scale, batch=1):
+    bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size], inorder)
+    bo = oracle.Box.make((0, 0, 0), [s - 1 for s in size], outorder)
+    return oracle.fft3d_c2c(bi, bo, x, sign, scale, batch)
+
+
+def test_config1_64cubed_double(hmi, cuda, oracle):
+    """64^3 complex<double>, input order (0,1,2), output (2,1,0), U[-1,1] seed 0;
+    forward scale::none then backward scale::full."""
+    import torch
+    size = (64, 64, 64)
+    n = 64 ** 3
+    inbox = hmi.box3d([0, 0, 0], [63, 63, 63], [0, 1, 2])
+    outbox = hmi.box3d([0, 0, 0], [63, 63, 63], [2, 1, 0])
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    x = fc.uniform_complex(n, 0)
+    tx = fc.to_gpu(x, np.complex128)
+    ty = torch.empty_like(tx)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=tx.dtype, device=cuda)
+    plan.forward_buffered(tx, ty, ws, hmi.scale.none)
+    ref = oracle_c2c(oracle, size, (0, 1, 2), (2, 1, 0), x, -1, 1.0)
+    err = fc.rel_l2(fc.to_host(ty), ref)
+    print(f"64^3 z2z forward rel L2 = {err:.3e}")
+    assert err <= 1e-13
+    tz = torch.empty_like(tx)
+    plan.backward_buffered(ty, tz, ws, hmi.scale.full)
+    err_rt = fc.rel_l2(fc.to_host(tz), x)
+    print(f"64^3 z2z round trip rel L2 = {err_rt:.3e}")
+    assert err_rt <= 1e-13
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("size", [(6, 7, 8), (7, 8, 6), (43, 76, 24), (1, 1, 1), (1, 16, 1), (32, 1, 12)])
+def test_reference_sizes(hmi, cuda, oracle, dtype, size):
+    for inorder, outorder in [((0, 1, 2), (0, 1, 2)), ((0, 1, 2), (2, 1, 0)), ((1, 2, 0), (0, 2, 1))]:
+        x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=11)
+        ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+        err = fc.rel_l2(y, ref)
+        assert err <= fc.TOL[dtype], (size, inorder, outorder, err)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("oi", range(6))
+@pytest.mark.parametrize("oo", range(6))
+def test_all_order_pairs(hmi, cuda, oracle, dtype, oi, oo):
+    size = (16, 12, 10)
+    inorder, outorder = fc.ORDERS[oi], fc.ORDERS[oo]
+    for in_place in (False, True):
+        x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=oi * 6 + oo, in_place=in_place)
+        ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+        err = fc.rel_l2(y, ref)
+        assert err <= fc.TOL[dtype], (inorder, outorder, in_place, err)
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_batch_backward_scales(hmi, cuda, oracle, dtype):
+    """8x9x10, batch 5, all three scalings, backward direction, internal workspace."""
+    size = (8, 9, 10)
+    n = 720
+    for scaling, s in ((0, 1.0), (1, 1.0 / n), (2, 1.0 / np.sqrt(n))):
+        x, y, _ = run_c2c(hmi, cuda, size, (0, 1, 2), (2, 0, 1), dtype, seed=5, batch=5, use_ws=False,
+                          scaling=scaling, direction="backward")
+        ref = oracle_c2c(oracle, size, (2, 0, 1), (0, 1, 2), x, +1, s, batch=5)
+        assert fc.rel_l2(y, ref) <= fc.TOL[dtype], scaling
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("size", [(64, 64, 64), (128, 32, 256), (512, 8, 4), (100, 45, 27), (96, 80, 63)])
+def test_fast_and_generic_lengths(hmi, cuda, oracle, dtype, size):
+    for inorder, outorder in [((0, 1, 2), (1, 2, 0)), ((2, 1, 0), (0, 1, 2))]:
+        for in_place in (False, True):
+            x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=3, in_place=in_place)
+            ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+            err = fc.rel_l2(y, ref)
+            assert err <= fc.TOL[dtype], (size, inorder, outorder, in_place, err)
+
+
+@pytest.mark.parametrize("rdt,cdt", [(np.float32, np.complex64), (np.float64, np.complex128)])
+def test_real_input_and_real_output(hmi, cuda, oracle, rdt, cdt):
+    """fft3d::forward(real in, complex out) and backward(complex in, real out),
+    including the in-place complex -> real call of test_fft3d_arrays."""
+    import torch
+    size = (12, 10, 16)
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [11, 9, 15], (0, 1, 2))
+    outbox = hmi.box3d([0, 0, 0], [11, 9, 15], (2, 1, 0))
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    rng = np.random.default_rng(8)
+    xr = rng.uniform(-1, 1, n)
+    t_in = torch.from_numpy(xr.astype(rdt)).to(cuda)
+    t_out = torch.empty(n, dtype=torch.complex64 if cdt == np.complex64 else torch.complex128, device=cuda)
+    plan.forward(t_in, t_out)
+    bi = oracle.Box.make((0, 0, 0), (11, 9, 15), (0, 1, 2))
+    bo = oracle.Box.make((0, 0, 0), (11, 9, 15), (2, 1, 0))
+    ref = oracle.fft3d_real_c2c(bi, bo, xr)
+    tol = fc.TOL["complex64" if cdt == np.complex64 else "complex128"]
+    assert fc.rel_l2(fc.to_host(t_out), ref) <= tol
+    # complex -> real (truncation of the imaginary part), out of place and in place
+    back = torch.empty(n, dtype=t_in.dtype, device=cuda)
+    plan.backward(t_out, back, hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(back), xr) <= tol
+    buf = t_out.clone()
+    plan.backward(buf, buf.view(t_in.dtype)[:n], hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(buf.view(t_in.dtype)[:n]), xr) <= tol
+
+
+def test_stream_binding(hmi, cuda, oracle):
+    """The plan runs on the bound stream (fft3d(gpu_stream, ...))."""
+    import torch
+    size = (32, 32, 32)
+    s = torch.cuda.Stream()
+    inbox = hmi.box3d([0, 0, 0], [31, 31, 31])
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, inbox)
+    plan.set_stream(s)
+    x = fc.uniform_complex(32 ** 3, 4)
+    tx = fc.to_gpu(x, np.complex128)
+    ty = torch.empty_like(tx)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        plan.forward(tx, ty)
+    s.synchronize()
+    ref = oracle_c2c(oracle, size, (0, 1, 2), (0, 1, 2), x, -1, 1.0)
+    assert fc.rel_l2(ty.cpu().numpy(), ref) <= 1e-13

@@ -31,6 +31,12 @@ class heffte_input_error(Exception):
 
 
 def _load():
+    # torch ships its own libamdhip64.so.7; loading torch first makes the library
+    # bind to that same HIP runtime (one runtime per process, shared device memory).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIBRARY_PATH):
         raise ImportError(
             f"{LIBRARY_PATH} is missing: build it with `python heffte_amd/build.py` "

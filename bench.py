@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-rocfft", action="store_true", help="skip the rocFFT comparison")
     p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return p.parse_args()
 
 
@@ -140,6 +142,17 @@ def main():
                      "kernel": f"fft_uniform {n}-point pass", "avg_launch_ms": round(avg_launch_ms, 4),
                      "bytes_per_launch": bytes_per_launch},
     }
+
+    if args.calibrate_copy:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+        # calibrate the counters read for the FFT passes in the same profiled run
+        src = torch.empty(N, dtype=cdt, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = N * esize
+        del src, dst
 
     if not args.no_error:
         # error of one forward transform against the FP64 path (itself pinned to the oracle by the tests)

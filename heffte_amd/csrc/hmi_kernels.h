@@ -20,7 +20,9 @@ struct fast_kernel_info {
     fast_launcher launch;
 };
 
-const fast_kernel_info *find_fast_kernel(int n, bool dbl);
+// wide = the pass walks the column axis on load or store (needs 128-byte column
+// chunks); otherwise the variant with the fewest lines per block is returned.
+const fast_kernel_info *find_fast_kernel(int n, bool dbl, bool wide);
 
 hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,
                           hipStream_t s);

@@ -105,11 +105,47 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     if (a.store_colfast) { l2 = tid % T; t2 = tid / T; } else { l2 = tid / P; t2 = tid % P; }
 
     const R sg = a.conj ? R(-1) : R(1);
+    const C *__restrict__ tw = static_cast<const C *>(a.tw);
+    // post-twiddles w_{2N}^k of the real-data passes follow the stage twiddles
+    const C *__restrict__ tw_half = tw + RL::twoff(RL::count);
     C v[E];
     {
         const bool valid = (c0 + l) < a.ncol;
         const long long base = (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2 + (long long)(c0 + l) * a.in_sc;
-        if (a.in_real) {
+        if (a.mode == kModeR2C) {
+            // real line of 2N values read as N complex pairs z[m] = x[2m] + i x[2m+1]
+            const R *__restrict__ src = static_cast<const R *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const long long m2 = 2LL * (t + P * e);
+                v[e] = valid ? cmk<C>(src[base + m2 * a.in_sa], src[base + (m2 + 1) * a.in_sa]) : cmk<C>(R(0), R(0));
+            }
+        } else if (a.mode == kModeC2R) {
+            // half spectrum X[0..N] of a real line of 2N values -> Z[k] = (X[k] + conj X[N-k])
+            //   + i w^{-k} (X[k] - conj X[N-k]), the spectrum of z[m] = x[2m] + i x[2m+1];
+            // the imaginary parts of X[0] and X[N] do not contribute to a real signal
+            const C *__restrict__ src = static_cast<const C *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                lds[l * RS + padj<PS>(t + P * e)] =
+                    valid ? src[base + (long long)(t + P * e) * a.in_sa] : cmk<C>(R(0), R(0));
+            if (t == 0) lds[l * RS + padj<PS>(N)] = valid ? src[base + (long long)N * a.in_sa] : cmk<C>(R(0), R(0));
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + P * e;
+                C xk = lds[l * RS + padj<PS>(k)];
+                C xm = lds[l * RS + padj<PS>(N - k)];
+                if (k == 0) { xk.y = R(0); xm.y = R(0); }
+                const C w = tw_half[k];  // exp(-i pi k / N); w^{-k} = conj(w)
+                const C sum = cmk<C>(xk.x + xm.x, xk.y - xm.y);
+                const C dif = cmk<C>(xk.x - xm.x, xk.y + xm.y);
+                const C wd = cmk<C>(w.x * dif.x + w.y * dif.y, w.x * dif.y - w.y * dif.x);  // conj(w) * dif
+                const C z = cmk<C>(sum.x - wd.y, sum.y + wd.x);                             // sum + i wd
+                v[e] = cmk<C>(z.x, -z.y);  // inverse DFT through the forward stages: conjugate in and out
+            }
+            __syncthreads();
+        } else if (a.in_real) {
             const R *__restrict__ src = static_cast<const R *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++)
@@ -133,7 +169,6 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
         }
     }
 
-    const C *__restrict__ tw = static_cast<const C *>(a.tw);
     run_stages<0, RL, E, P, RS, PS>(v, lds, tw, l, t, l2, t2);
     if constexpr (RL::count == 1) {
         if (a.load_colfast != a.store_colfast) {
@@ -142,29 +177,56 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; e++) v[e] = lds[l2 * RS + padj<PS>(t2 + P * e)];
+            __syncthreads();
         }
     }
 
-    if ((c0 + l2) < a.ncol) {
-        const R s = (R)a.scale;
-        const long long base =
-            (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)(c0 + l2) * a.out_sc;
-        if (a.out_real) {
-            R *__restrict__ dst = static_cast<R *>(a.out);
+    const bool svalid = (c0 + l2) < a.ncol;
+    const R s = (R)a.scale;
+    const long long obase = (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)(c0 + l2) * a.out_sc;
+    if (a.mode == kModeR2C) {
+        // Z = DFT_N(z); X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[N-k]) / 2, O = -i (Z[k] - conj Z[N-k]) / 2
 #pragma unroll
-            for (int e = 0; e < E; e++) dst[base + (long long)(t2 + P * e) * a.out_sa] = v[e].x * s;
-        } else if (a.store_len < N) {
-            C *__restrict__ dst = static_cast<C *>(a.out);
+        for (int e = 0; e < E; e++) lds[l2 * RS + padj<PS>(t2 + P * e)] = v[e];
+        __syncthreads();
+        C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
-            for (int e = 0; e < E; e++)
-                if (t2 + P * e < a.store_len)
-                    dst[base + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
-        } else {
-            C *__restrict__ dst = static_cast<C *>(a.out);
-#pragma unroll
-            for (int e = 0; e < E; e++)
-                dst[base + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+        for (int e = 0; e < E; e++) {
+            const int k = t2 + P * e;
+            const C zk = v[e];
+            const C zm = lds[l2 * RS + padj<PS>(k == 0 ? 0 : N - k)];
+            const C ev = cmk<C>(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
+            const C od = cmk<C>(R(0.5) * (zk.y + zm.y), R(-0.5) * (zk.x - zm.x));
+            const C w = tw_half[k];
+            const C x = cadd(ev, cmul(w, od));
+            if (svalid) dst[obase + (long long)k * a.out_sa] = cmk<C>(x.x * s, x.y * s);
+            if (k == 0 && svalid) dst[obase + (long long)N * a.out_sa] = cmk<C>((ev.x - od.x) * s, (ev.y - od.y) * s);
         }
+    } else if (!svalid) {
+        // column beyond the box: nothing to store
+    } else if (a.mode == kModeC2R) {
+        R *__restrict__ dst = static_cast<R *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const long long m2 = 2LL * (t2 + P * e);
+            dst[obase + m2 * a.out_sa] = v[e].x * s;
+            dst[obase + (m2 + 1) * a.out_sa] = -v[e].y * s;
+        }
+    } else if (a.out_real) {
+        R *__restrict__ dst = static_cast<R *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++) dst[obase + (long long)(t2 + P * e) * a.out_sa] = v[e].x * s;
+    } else if (a.store_len < N) {
+        C *__restrict__ dst = static_cast<C *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            if (t2 + P * e < a.store_len)
+                dst[obase + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+    } else {
+        C *__restrict__ dst = static_cast<C *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            dst[obase + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
     }
 }
 
@@ -320,7 +382,7 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
 // ----------------------------------------------------------- fast registry
 
 #define HMI_FAST(TYPE, N, E, T, PS, ...)                                                                  \
-    static hipError_t launch_##TYPE##_##N(const pass_args &a, long long nblocks, hipStream_t s) {          \
+    static hipError_t launch_##TYPE##_##N##_##T(const pass_args &a, long long nblocks, hipStream_t s) {    \
         fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS><<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a); \
         return hipGetLastError();                                                                          \
     }
@@ -333,9 +395,13 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
     X(float, 128, 16, 32, 4, 16, 8)              \
     X(float, 256, 16, 16, 4, 16, 16)             \
     X(float, 512, 8, 16, 3, 8, 8, 8)             \
+    X(float, 512, 8, 4, 3, 8, 8, 8)              \
     X(float, 1024, 16, 8, 4, 16, 8, 8)           \
+    X(float, 1024, 16, 4, 4, 16, 8, 8)           \
     X(float, 2048, 16, 4, 4, 8, 16, 16)          \
+    X(float, 2048, 16, 2, 4, 8, 16, 16)          \
     X(float, 4096, 16, 4, 4, 16, 16, 16)         \
+    X(float, 4096, 16, 1, 4, 16, 16, 16)         \
     X(float, 12, 12, 128, 4, 4, 3)               \
     X(float, 20, 20, 64, 4, 4, 5)                \
     X(float, 24, 12, 64, 4, 3, 4, 2)             \
@@ -356,21 +422,27 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
     X(double, 128, 16, 16, 4, 16, 8)             \
     X(double, 256, 16, 8, 4, 16, 16)             \
     X(double, 512, 8, 8, 3, 8, 8, 8)             \
+    X(double, 512, 8, 4, 3, 8, 8, 8)             \
     X(double, 1024, 16, 4, 4, 16, 8, 8)          \
+    X(double, 1024, 16, 2, 4, 16, 8, 8)          \
     X(double, 2048, 16, 2, 4, 8, 16, 16)         \
-    X(double, 4096, 16, 2, 4, 16, 16, 16)
+    X(double, 4096, 16, 2, 4, 16, 16, 16)        \
+    X(double, 4096, 16, 1, 4, 16, 16, 16)
 
 HMI_FAST_TABLE(HMI_FAST)
 
 #define HMI_FAST_ENTRY(TYPE, N, E, T, PS, ...) \
-    {sizeof(TYPE) == 8, N, E, T, {__VA_ARGS__}, radices<__VA_ARGS__>::count, &launch_##TYPE##_##N},
+    {sizeof(TYPE) == 8, N, E, T, {__VA_ARGS__}, radices<__VA_ARGS__>::count, &launch_##TYPE##_##N##_##T},
 
 static const fast_kernel_info kFast[] = {HMI_FAST_TABLE(HMI_FAST_ENTRY)};
 
-const fast_kernel_info *find_fast_kernel(int n, bool dbl) {
-    for (const auto &k : kFast)
-        if (k.n == n && k.dbl == dbl) return &k;
-    return nullptr;
+const fast_kernel_info *find_fast_kernel(int n, bool dbl, bool wide) {
+    const fast_kernel_info *best = nullptr;
+    for (const auto &k : kFast) {
+        if (k.n != n || k.dbl != dbl) continue;
+        if (!best || (wide ? k.T > best->T : k.T < best->T)) best = &k;
+    }
+    return best;
 }
 
 hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,

@@ -15,6 +15,12 @@ namespace hmi {
 
 enum : int { kMaxStages = 24 };
 
+// Pass modes of the uniform kernel.  The real-data modes run a complex FFT of
+// half the line length: kModeR2C reads 2N reals as N complex pairs and unpacks
+// the half spectrum (N + 1 bins) after the FFT; kModeC2R packs the N + 1 bins
+// into N complex values before an inverse FFT and writes 2N reals.
+enum : int { kModeC2C = 0, kModeR2C = 1, kModeC2R = 2 };
+
 struct pass_args {
     const void *in;
     void *out;
@@ -32,6 +38,7 @@ struct pass_args {
     int out_real;  // 1: store only the real part
     int store_len; // positions >= store_len along the FFT axis are not stored (r2c half spectrum)
     int herm_load; // 1: positions > n/2 are read as conj(x[n - pos]) (c2r Hermitian extension)
+    int mode;      // kModeC2C, kModeR2C or kModeC2R (uniform kernel only)
     double scale;  // multiplies every stored value (cast to the pass precision)
 };
 

@@ -176,6 +176,7 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                     }
                     p.n = cp.identity ? (int)dims[cp.axis] : (int)nfull[cp.axis];
                     if (r2c && cp.axis == r2c_dir_ && !cp.identity) {
+                        p.real_axis = true;
                         if (fwd) p.store_len = (int)odims[r2c_dir_];
                         else p.herm_load = 1;
                     }
@@ -277,7 +278,18 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
 
 void fft_plan::configure(pass &p, bool dbl) const {
     const size_t elem = dbl ? 16 : 8;
-    p.fast = p.identity ? nullptr : find_fast_kernel(p.n, dbl);
+    const bool wide = p.load_colfast || p.store_colfast;
+    p.mode = kModeC2C;
+    if (p.real_axis && p.n % 2 == 0) {
+        // real line of n values through a complex FFT of n / 2 (pack / unpack in the same kernel)
+        p.fast = find_fast_kernel(p.n / 2, dbl, wide);
+        if (p.fast) {
+            p.mode = p.herm_load ? kModeC2R : kModeR2C;
+            p.lines = p.fast->T;
+            return;
+        }
+    }
+    p.fast = p.identity ? nullptr : find_fast_kernel(p.n, dbl, wide);
     if (p.fast) {
         p.lines = p.fast->T;
         return;
@@ -322,7 +334,7 @@ template <typename R> void root(long long e, long long m, R &re, R &im) {
 }
 
 template <typename R>
-std::vector<R> make_table(const int *radix, int nstages, bool with_roots) {
+std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int real_n) {
     std::vector<R> t;
     long long ls = 1;
     for (int s = 0; s < nstages; s++) {
@@ -346,14 +358,23 @@ std::vector<R> make_table(const int *radix, int nstages, bool with_roots) {
                 t.push_back(im);
             }
         }
+    // w_{real_n}^k, k < real_n / 2: the unpack / pack twiddles of the real-data modes
+    for (int k = 0; k < real_n / 2; k++) {
+        R re, im;
+        root<R>(k, real_n, re, im);
+        t.push_back(re);
+        t.push_back(im);
+    }
     if (t.empty()) { t.push_back(R(1)); t.push_back(R(0)); }
     return t;
 }
 
 }  // namespace
 
-const twiddles &fft_plan::get_twiddles(int n, bool dbl, const pass &p) {
-    const long long key = ((long long)n << 3) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0);
+const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
+    const int n = p.n;
+    const long long key = ((long long)n << 5) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
+                          ((long long)p.mode << 3) | (p.fast ? (long long)p.fast->T << 40 : 0);
     std::lock_guard<std::mutex> lock(mtx_);
     auto it = tw_.find(key);
     if (it != tw_.end()) return it->second;
@@ -361,12 +382,12 @@ const twiddles &fft_plan::get_twiddles(int n, bool dbl, const pass &p) {
     const int ns = p.fast ? p.fast->nstages : p.gen.nstages;
     twiddles t;
     if (dbl) {
-        std::vector<double> v = make_table<double>(radix, ns, !p.fast);
+        std::vector<double> v = make_table<double>(radix, ns, !p.fast, p.mode != kModeC2C ? n : 0);
         t.bytes = v.size() * sizeof(double);
         check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
         check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
     } else {
-        std::vector<float> v = make_table<float>(radix, ns, !p.fast);
+        std::vector<float> v = make_table<float>(radix, ns, !p.fast, p.mode != kModeC2C ? n : 0);
         t.bytes = v.size() * sizeof(float);
         check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
         check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
@@ -416,7 +437,7 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         };
         a.in = bufptr(p.src);
         a.out = bufptr(p.dst);
-        const twiddles &tw = get_twiddles(p.n, dbl, p);
+        const twiddles &tw = get_twiddles(dbl, p);
         a.tw = tw.dev;
         a.in_sa = p.lin.stride[p.axis];
         a.in_sc = p.lin.stride[p.col];
@@ -437,6 +458,7 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.out_real = p.out_real;
         a.store_len = p.store_len;
         a.herm_load = p.herm_load;
+        a.mode = p.mode;
         a.scale = p.last ? scale : 1.0;
         const long long nblocks = (long long)a.ncolblk * a.nb1 * batch;
         if (nblocks > 0x7fffffffLL) throw std::runtime_error("grid too large");

@@ -57,6 +57,8 @@ struct pass {
     int in_real = 0, out_real = 0, herm_load = 0, store_len = 0;
     bool last = false;
     bool identity = false;  // copy pass (no FFT stages), used when no in-place schedule exists
+    bool real_axis = false; // the r2c / c2r pass of a fft3d_r2c plan
+    int mode = 0;           // kModeC2C, or kModeR2C / kModeC2R when the half-length kernel runs it
     // launch configuration
     const fast_kernel_info *fast = nullptr;
     generic_plan gen{};
@@ -99,7 +101,7 @@ class fft_plan {
   private:
     exec_plan build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const;
     void configure(pass &p, bool dbl) const;
-    const twiddles &get_twiddles(int n, bool dbl, const pass &p);
+    const twiddles &get_twiddles(bool dbl, const pass &p);
 
     kind kind_;
     box3 inbox_, outbox_;

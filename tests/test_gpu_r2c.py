@@ -47,7 +47,8 @@ def test_known_answers_2x3x4(hmi, cuda, golden, dtype, axis):
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-@pytest.mark.parametrize("size", [(6, 7, 8), (7, 8, 6), (16, 10, 9), (64, 32, 16), (33, 20, 12)])
+@pytest.mark.parametrize("size", [(6, 7, 8), (7, 8, 6), (16, 10, 9), (64, 32, 16), (33, 20, 12), (1024, 8, 6),
+                                  (128, 96, 24)])
 @pytest.mark.parametrize("r2c_dir", [0, 1, 2])
 def test_r2c_c2r(hmi, cuda, oracle, dtype, size, r2c_dir):
     import torch
@@ -76,3 +77,32 @@ def test_r2c_c2r(hmi, cuda, oracle, dtype, size, r2c_dir):
         back2 = torch.empty_like(tx)
         plan.backward(ty, back2)
         assert fc.rel_l2(fc.to_host(back2), ref_b) <= tol
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_r2c_large_round_trip(hmi, cuda, dtype):
+    """1024 x 1024 x 64 real (the r2c along the first, contiguous axis of the 1024 x 1024 x 512
+    configuration, fewer planes): c2r(r2c(x)) / N == x and Parseval on the half spectrum."""
+    import torch
+    rdt, tol = RDT[dtype]
+    size = (1024, 1024, 64)
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size])
+    outbox = inbox.r2c(0)
+    plan = hmi.fft3d_r2c(hmi.backend.mi355, inbox, outbox, 0)
+    g = torch.Generator(device=cuda)
+    g.manual_seed(2)
+    x = torch.randn(inbox.count(), dtype=torch.float32 if rdt == np.float32 else torch.float64, device=cuda,
+                    generator=g)
+    y = torch.empty(outbox.count(), dtype=torch.complex64 if rdt == np.float32 else torch.complex128, device=cuda)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=y.dtype, device=cuda)
+    plan.forward_buffered(x, y, ws)
+    z = torch.empty_like(x)
+    plan.backward_buffered(y, z, ws, hmi.scale.full)
+    err = (torch.linalg.vector_norm((z - x).double()) / torch.linalg.vector_norm(x.double())).item()
+    assert err <= tol, err
+    # Parseval: sum |x|^2 = (|X_0|^2 + 2 sum_{0<k<n/2} |X_k|^2 + |X_{n/2}|^2) / N along the halved axis
+    Y = y.view(size[2], size[1], size[0] // 2 + 1).double() if False else y.view(size[2], size[1], size[0] // 2 + 1)
+    p = (Y.abs() ** 2).double()
+    tot = p[..., 0].sum() + p[..., -1].sum() + 2 * p[..., 1:-1].sum()
+    lhs = (x.double() ** 2).sum() * inbox.count()
+    assert abs((tot - lhs) / lhs).item() <= 10 * tol

@@ -24,6 +24,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "512³ c2c fwd+bwd ms & %HBM roofline; split-FP16 vs FP32 at ≤5e-6 rel L2"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per pass launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the value the
+# same counter reads for a plain 1 GiB copy (profiles/fft_uniform_512_c64_hbm_traffic.md)
+MEASURED_TRAFFIC = {(512, "complex64"): (2147678208, "profiles/fft_uniform_512_c64_hbm_traffic.md")}
 
 
 def parse():
@@ -138,9 +141,11 @@ def main():
                    "n": n, "batch": 1, "parallelism": "single rank, 1 GPU"},
         "gflops_5nlog2n": round(gflops, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"fft_uniform {n}-point pass", "avg_launch_ms": round(avg_launch_ms, 4),
-                     "bytes_per_launch": bytes_per_launch},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": MEASURED_TRAFFIC.get((n, args.dtype), (None,))[0],
+                     "traffic_source": MEASURED_TRAFFIC.get((n, args.dtype), (None, None))[1],
+                     "kernel": f"fft_uniform {n}-point pass (all 6 launches of a pair)",
+                     "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": bytes_per_launch},
     }
 
     if args.calibrate_copy:

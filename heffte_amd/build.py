@@ -65,5 +65,21 @@ def build(verbose=False):
     return LIB
 
 
+FRONTEND_TEST_SRC = os.path.join(HERE, "..", "tests", "cpp", "test_frontend.cpp")
+FRONTEND_TEST_BIN = os.path.join(HERE, "..", "tests", "cpp", "test_frontend.bin")
+
+
+def build_frontend_test():
+    """Compiles the C++ front-end test program (include/heffte_mi355.hpp) against the library."""
+    cmd = [hipcc(), "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-DHEFFTE_MI355_NO_MPI",
+           "-I" + os.path.join(HERE, "..", "include"), FRONTEND_TEST_SRC, "-o", FRONTEND_TEST_BIN,
+           "-L" + HERE, "-lheffte_mi355", "-Wl,-rpath,$ORIGIN/../../heffte_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"front-end test build failed:\n{r.stderr}")
+    return FRONTEND_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_frontend_test())

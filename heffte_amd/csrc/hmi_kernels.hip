@@ -145,6 +145,32 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
                 v[e] = cmk<C>(z.x, -z.y);  // inverse DFT through the forward stages: conjugate in and out
             }
             __syncthreads();
+        } else if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
+            const bool sine = (a.mode == kModeDST2);
+            const R *__restrict__ src = static_cast<const R *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int n = t + P * e;
+                const int j = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
+                R x = valid ? src[base + (long long)j * a.in_sa] : R(0);
+                if (sine && (j & 1)) x = -x;
+                v[e] = cmk<C>(x, R(0));
+            }
+        } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
+            const bool sine = (a.mode == kModeDST3);
+            const R *__restrict__ src = static_cast<const R *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int k = t + P * e;
+                const int jk = sine ? N - 1 - k : k;
+                const int jm = sine ? k - 1 : N - k;
+                const R xk = valid ? src[base + (long long)jk * a.in_sa] : R(0);
+                const R xm = (valid && k > 0) ? src[base + (long long)jm * a.in_sa] : R(0);
+                const C w = tw_half[k];  // exp(-i pi k / 2N)
+                // W = conj(w) (xk - i xm); the inverse DFT runs as conj(DFT(conj W))
+                const C W = cmk<C>(w.x * xk - w.y * xm, -w.x * xm - w.y * xk);
+                v[e] = cmk<C>(W.x, -W.y);
+            }
         } else if (a.in_real) {
             const R *__restrict__ src = static_cast<const R *>(a.in);
 #pragma unroll
@@ -204,6 +230,27 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
         }
     } else if (!svalid) {
         // column beyond the box: nothing to store
+    } else if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
+        const bool sine = (a.mode == kModeDST2);
+        R *__restrict__ dst = static_cast<R *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int k = t2 + P * e;
+            const C w = tw_half[k];
+            const R y = R(2) * (w.x * v[e].x - w.y * v[e].y);
+            dst[obase + (long long)(sine ? N - 1 - k : k) * a.out_sa] = y * s;
+        }
+    } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
+        const bool sine = (a.mode == kModeDST3);
+        R *__restrict__ dst = static_cast<R *>(a.out);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int m = t2 + P * e;
+            const int j = (m < (N + 1) / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
+            R y = R(2) * v[e].x;
+            if (sine && (j & 1)) y = -y;
+            dst[obase + (long long)j * a.out_sa] = y * s;
+        }
     } else if (a.mode == kModeC2R) {
         R *__restrict__ dst = static_cast<R *>(a.out);
 #pragma unroll
@@ -270,7 +317,19 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
         C x = cmk<C>(R(0), R(0));
         if (c0 + l < a.ncol) {
             const long long off = ibase + (long long)l * a.in_sc + (long long)j * a.in_sa;
-            if (a.in_real) {
+            const R *rsrc = static_cast<const R *>(a.in);
+            const long long lbase = ibase + (long long)l * a.in_sc;
+            if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
+                const int jj = (j < (N + 1) / 2) ? 2 * j : 2 * (N - 1 - j) + 1;
+                x.x = rsrc[lbase + (long long)jj * a.in_sa];
+                if (a.mode == kModeDST2 && (jj & 1)) x.x = -x.x;
+            } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
+                const bool sine = (a.mode == kModeDST3);
+                const R xk = rsrc[lbase + (long long)(sine ? N - 1 - j : j) * a.in_sa];
+                const R xm = j > 0 ? rsrc[lbase + (long long)(sine ? j - 1 : N - j) * a.in_sa] : R(0);
+                const C w = static_cast<const C *>(a.tw)[g.postoff + j];
+                x = cmk<C>(w.x * xk - w.y * xm, w.x * xm + w.y * xk);  // conj of conj(w) (xk - i xm)
+            } else if (a.in_real) {
                 x.x = static_cast<const R *>(a.in)[off];
             } else if (a.herm_load && j > N / 2) {
                 x = static_cast<const C *>(a.in)[ibase + (long long)l * a.in_sc + (long long)(N - j) * a.in_sa];
@@ -335,7 +394,17 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
         if (c0 + l >= a.ncol || j >= a.store_len) continue;
         const C y = src[l * N + j];
         const long long off = obase + (long long)l * a.out_sc + (long long)j * a.out_sa;
-        if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
+        if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
+            const C w = static_cast<const C *>(a.tw)[g.postoff + j];
+            const int k = (a.mode == kModeDST2) ? N - 1 - j : j;
+            static_cast<R *>(a.out)[obase + (long long)l * a.out_sc + (long long)k * a.out_sa] =
+                R(2) * (w.x * y.x - w.y * y.y) * sc;
+        } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
+            const int jj = (j < (N + 1) / 2) ? 2 * j : 2 * (N - 1 - j) + 1;
+            R r = R(2) * y.x;
+            if (a.mode == kModeDST3 && (jj & 1)) r = -r;
+            static_cast<R *>(a.out)[obase + (long long)l * a.out_sc + (long long)jj * a.out_sa] = r * sc;
+        } else if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
         else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
     }
 }

@@ -19,7 +19,15 @@ enum : int { kMaxStages = 24 };
 // half the line length: kModeR2C reads 2N reals as N complex pairs and unpacks
 // the half spectrum (N + 1 bins) after the FFT; kModeC2R packs the N + 1 bins
 // into N complex values before an inverse FFT and writes 2N reals.
-enum : int { kModeC2C = 0, kModeR2C = 1, kModeC2R = 2 };
+//
+// The real-to-real modes compute one axis of rtransform<cos/sin> through an
+// N-point complex FFT (Makhoul's reordering):
+//   kModeDCT2 (forward cos): v[n] = x[2n], v[N-1-n] = x[2n+1]; Y[k] = 2 Re(e^{-i pi k/2N} V[k])
+//   kModeDCT3 (backward cos): W[k] = e^{i pi k/2N} (x[k] - i x[N-k]), w = IDFT(W);
+//                             y[2m] = 2 Re w[m], y[2m+1] = 2 Re w[N-1-m]
+//   kModeDST2 / kModeDST3: the same with the sine identities
+//     DST-II(x)[k] = DCT-II((-1)^n x[n])[N-1-k],  DST-III(x)[n] = (-1)^n DCT-III(x[N-1-.])[n]
+enum : int { kModeC2C = 0, kModeR2C = 1, kModeC2R = 2, kModeDCT2 = 3, kModeDCT3 = 4, kModeDST2 = 5, kModeDST3 = 6 };
 
 struct pass_args {
     const void *in;
@@ -50,6 +58,7 @@ struct generic_plan {
     int radix[kMaxStages];
     int twoff[kMaxStages];     // offset (complex units) of stage twiddles w_{Ls R}^{p k}
     int rootoff[kMaxStages];   // offset of w_R^j, j < R (used by radices without a hard-coded butterfly)
+    int postoff;               // offset of the real-data twiddles (modes other than kModeC2C)
 };
 
 }  // namespace hmi

@@ -85,7 +85,6 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
         if (!inbox.same_indexes(outbox))
             throw std::invalid_argument("inbox and outbox must hold the same indexes on a single rank");
     }
-    if (k == kind::cos || k == kind::sin) throw std::invalid_argument("r2r transforms are not available yet");
     // workspace: the largest over every variant the plan may run
     long long ws = 0;
     for (int dir : {-1, 1})
@@ -95,6 +94,7 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
                     if (k == kind::c2c && ir && dir == 1) continue;
                     if (k == kind::c2c && orl && dir == -1) continue;
                     if (k == kind::r2c && (ir != (dir == -1) || orl != (dir == 1))) continue;
+                    if ((k == kind::cos || k == kind::sin) && (!ir || !orl)) continue;
                     exec_plan p = build(dir, false, ir, orl, ip);
                     ws = std::max(ws, p.ws_elems);
                 }
@@ -110,6 +110,9 @@ double fft_plan::scale_factor(int scaling) const {
     // include/heffte_fft3d.h:577 (1 / N of the real-space world) and :545-547
     const double n = (double)inbox_.count();
     double s = 1.0 / n;
+    // rtransform of a GPU/stock backend: round trip 4 n per axis, 64 N in 3D
+    // (include/heffte_fft3d.h:635-650, the branch every non-FFTW r2r tag takes)
+    if (kind_ == kind::cos || kind_ == kind::sin) s /= 64.0;
     if (scaling == 0) return 1.0;
     if (scaling == 2) return std::sqrt(s);
     return s;
@@ -137,10 +140,12 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
     for (int d = 0; d < 3; d++) nfull[d] = inbox_.size(d);
 
     // axes to transform
+    const bool r2r = (kind_ == kind::cos || kind_ == kind::sin);
     std::vector<int> axes;
     if (r2c && fwd) axes.push_back(r2c_dir_);
     for (int d = 0; d < 3; d++)
-        if (nfull[d] > 1 && !(r2c && d == r2c_dir_)) axes.push_back(d);
+        // a length-1 cosine/sine transform still multiplies by 2, so r2r keeps every axis
+        if ((nfull[d] > 1 || r2r) && !(r2c && d == r2c_dir_)) axes.push_back(d);
     if (r2c && !fwd) axes.push_back(r2c_dir_);
     if (axes.empty()) axes.push_back(0);
 
@@ -193,12 +198,14 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                         p.dst = (bcode % 2) ? buf::ws : buf::out;
                         bcode /= 2;
                     }
-                    p.in_real = (i == 0 && in_real) ? 1 : 0;
-                    p.out_real = (p.last && out_real) ? 1 : 0;
+                    p.in_real = (r2r || (i == 0 && in_real)) ? 1 : 0;
+                    p.out_real = (r2r || (p.last && out_real)) ? 1 : 0;
+                    if (r2r && !cp.identity)
+                        p.mode = (kind_ == kind::cos) ? (fwd ? kModeDCT2 : kModeDCT3) : (fwd ? kModeDST2 : kModeDST3);
                     // an intermediate in OUT must fit the output buffer as complex data
                     if (!p.last && p.dst == buf::out) {
                         const long long cnt = odims[0] * odims[1] * odims[2];
-                        if (out_real || cnt > bout.count()) { ok = false; break; }
+                        if ((out_real && !r2r) || cnt > bout.count()) { ok = false; break; }
                     }
                     if (p.dst == buf::ws) uses_ws = true;
                     // coalescing: pick the column axis
@@ -226,7 +233,8 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                                           (in_place && ((p.src == buf::in && p.dst == buf::out) ||
                                                         (p.src == buf::out && p.dst == buf::in)));
                     if (same_mem) {
-                        bool preserve = !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
+                        // r2r passes read and write reals: same element type on both sides
+                        bool preserve = (r2r || (!p.in_real && !p.out_real)) && !p.herm_load && p.store_len == p.n;
                         for (int d = 0; d < 3 && preserve; d++)
                             if (dims[d] != odims[d] || (dims[d] > 1 && p.lin.stride[d] != p.lout.stride[d]))
                                 preserve = false;
@@ -279,8 +287,17 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
 void fft_plan::configure(pass &p, bool dbl) const {
     const size_t elem = dbl ? 16 : 8;
     const bool wide = p.load_colfast || p.store_colfast;
-    p.mode = kModeC2C;
-    if (p.real_axis && p.n % 2 == 0) {
+    if (p.mode >= kModeDCT2) {
+        // r2r axis: an n-point complex FFT between the Makhoul pack and unpack
+        p.fast = find_fast_kernel(p.n, dbl, wide);
+        if (p.fast) {
+            p.lines = p.fast->T;
+            return;
+        }
+    } else {
+        p.mode = kModeC2C;
+    }
+    if (p.mode == kModeC2C && p.real_axis && p.n % 2 == 0) {
         // real line of n values through a complex FFT of n / 2 (pack / unpack in the same kernel)
         p.fast = find_fast_kernel(p.n / 2, dbl, wide);
         if (p.fast) {
@@ -311,6 +328,7 @@ void fft_plan::configure(pass &p, bool dbl) const {
         g.rootoff[s] = off;
         if (!has_butterfly(rad[s])) off += rad[s];
     }
+    g.postoff = off;
     const size_t per_line = 2 * (size_t)p.n * elem;
     if (per_line > (size_t)kGenericMaxLds)
         throw std::runtime_error("transform length " + std::to_string(p.n) + " exceeds the single-pass LDS capacity");
@@ -334,7 +352,7 @@ template <typename R> void root(long long e, long long m, R &re, R &im) {
 }
 
 template <typename R>
-std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int real_n) {
+std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int tail_count, long long tail_mod) {
     std::vector<R> t;
     long long ls = 1;
     for (int s = 0; s < nstages; s++) {
@@ -358,10 +376,10 @@ std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int re
                 t.push_back(im);
             }
         }
-    // w_{real_n}^k, k < real_n / 2: the unpack / pack twiddles of the real-data modes
-    for (int k = 0; k < real_n / 2; k++) {
+    // w_{tail_mod}^k, k < tail_count: the pack / unpack twiddles of the real-data modes
+    for (int k = 0; k < tail_count; k++) {
         R re, im;
-        root<R>(k, real_n, re, im);
+        root<R>(k, tail_mod, re, im);
         t.push_back(re);
         t.push_back(im);
     }
@@ -373,7 +391,7 @@ std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int re
 
 const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     const int n = p.n;
-    const long long key = ((long long)n << 5) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
+    const long long key = ((long long)n << 8) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
                           ((long long)p.mode << 3) | (p.fast ? (long long)p.fast->T << 40 : 0);
     std::lock_guard<std::mutex> lock(mtx_);
     auto it = tw_.find(key);
@@ -381,13 +399,18 @@ const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     const int *radix = p.fast ? p.fast->radix : p.gen.radix;
     const int ns = p.fast ? p.fast->nstages : p.gen.nstages;
     twiddles t;
+    // r2c / c2r: w_n^k for k < n/2 (n = real line length); r2r: w_{4n}^k for k < n
+    int tail_count = 0;
+    long long tail_mod = 1;
+    if (p.mode == kModeR2C || p.mode == kModeC2R) { tail_count = n / 2; tail_mod = n; }
+    if (p.mode >= kModeDCT2) { tail_count = n; tail_mod = 4LL * n; }
     if (dbl) {
-        std::vector<double> v = make_table<double>(radix, ns, !p.fast, p.mode != kModeC2C ? n : 0);
+        std::vector<double> v = make_table<double>(radix, ns, !p.fast, tail_count, tail_mod);
         t.bytes = v.size() * sizeof(double);
         check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
         check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
     } else {
-        std::vector<float> v = make_table<float>(radix, ns, !p.fast, p.mode != kModeC2C ? n : 0);
+        std::vector<float> v = make_table<float>(radix, ns, !p.fast, tail_count, tail_mod);
         t.bytes = v.size() * sizeof(float);
         check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
         check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
@@ -453,7 +476,8 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.ncolblk = (a.ncol + p.lines - 1) / p.lines;
         a.load_colfast = p.load_colfast;
         a.store_colfast = p.store_colfast;
-        a.conj = dir > 0 ? 1 : 0;
+        // r2r modes run their inverse through explicit conjugation inside the kernel
+        a.conj = (dir > 0 && p.mode < kModeDCT2) ? 1 : 0;
         a.in_real = p.in_real;
         a.out_real = p.out_real;
         a.store_len = p.store_len;

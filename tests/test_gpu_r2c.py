@@ -101,7 +101,7 @@ def test_r2c_large_round_trip(hmi, cuda, dtype):
     err = (torch.linalg.vector_norm((z - x).double()) / torch.linalg.vector_norm(x.double())).item()
     assert err <= tol, err
     # Parseval: sum |x|^2 = (|X_0|^2 + 2 sum_{0<k<n/2} |X_k|^2 + |X_{n/2}|^2) / N along the halved axis
-    Y = y.view(size[2], size[1], size[0] // 2 + 1).double() if False else y.view(size[2], size[1], size[0] // 2 + 1)
+    Y = y.view(size[2], size[1], size[0] // 2 + 1)
     p = (Y.abs() ** 2).double()
     tot = p[..., 0].sum() + p[..., -1].sum() + 2 * p[..., 1:-1].sum()
     lhs = (x.double() ** 2).sum() * inbox.count()

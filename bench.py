@@ -39,6 +39,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-rocfft", action="store_true", help="skip the rocFFT comparison")
     p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
+    p.add_argument("--no-batched", action="store_true", help="skip the 1000-transform mixed-size batch leg")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return p.parse_args()
@@ -68,6 +69,67 @@ def cpu_baseline(n):
     return {"value": round((t1 - t0) * 1e3, 1), "unit": "ms per fwd+bwd pair", "cores": threads, "kind": "port",
             "sample": f"one full {m}^3 forward+backward pair of the FP64 oracle (complex<double>, "
                       f"OpenMP, {threads} threads), (0,1,2)->(1,2,0), scale::symmetric"}
+
+
+SMOOTH = [16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64, 80, 96, 105, 112, 128]
+
+
+def batched_leg(heffte, torch, dev, steps):
+    """1,000 independent c2c complex<float> transforms, each side drawn from the smooth sizes
+    (seed 4, sides independent), values U[-1,1]: the batched work-list kernels replayed from a
+    hipGraph vs a per-transform launch loop of the plain fft3d plans.  One step = forward +
+    backward (scale::full) of the whole batch."""
+    import numpy as np
+    rng = np.random.default_rng(4)
+    sizes = [tuple(int(v) for v in rng.choice(SMOOTH, 3)) for _ in range(1000)]
+    counts = [a * b * c for a, b, c in sizes]
+    total = sum(counts)
+    plan = heffte.batch_plan(sizes)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4)
+    x = (torch.rand(total, dtype=torch.complex64, device=dev, generator=g) * 2 - (1 + 1j)).contiguous()
+    y = torch.empty_like(x)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def batched():
+        plan.forward(x, y)
+        plan.backward(y, y, heffte.scale.full)
+
+    ms_batched = timed(batched, steps)
+    yb = y.clone()
+    # per-transform loop of the plain kernels (one fft3d plan per shape, 3 launches per transform)
+    plans = {}
+    offs = np.cumsum([0] + counts)[:-1]
+    ws = torch.empty(max(counts), dtype=torch.complex64, device=dev)
+    for s in set(sizes):
+        b = heffte.box3d([0, 0, 0], [v - 1 for v in s])
+        plans[s] = heffte.fft3d(heffte.backend.mi355, b, b)
+    views = [(plans[s], x[o:o + c], y[o:o + c]) for s, o, c in zip(sizes, offs, counts)]
+
+    def loop():
+        for p, xi, yi in views:
+            p.forward_buffered(xi, yi, ws)
+        for p, xi, yi in views:
+            p.backward_buffered(yi, yi, ws, heffte.scale.full)
+
+    ms_loop = timed(loop, max(2, steps // 4))
+    # both return the input (round trip); report the larger deviation
+    err_b = (torch.linalg.vector_norm(yb - x) / torch.linalg.vector_norm(x)).item()
+    err_l = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+    return {"transforms": 1000, "elements": int(total), "distinct_shapes": len(plans),
+            "ms_per_pair_batched_graph": round(ms_batched, 3), "ms_per_pair_launch_loop": round(ms_loop, 3),
+            "speedup": round(ms_loop / ms_batched, 2),
+            "round_trip_rel_l2": max(err_b, err_l)}
 
 
 def main():
@@ -193,6 +255,13 @@ def main():
             torch.cuda.empty_cache()
         except Exception as e:  # pragma: no cover
             out["rocfft_torch_fftn_ms_per_pair"] = f"unavailable: {e}"
+
+    if not args.no_batched and n == 512:
+        del data, ws
+        torch.cuda.empty_cache()
+        out["batched_mixed_1000"] = batched_leg(heffte, torch, dev, max(3, args.steps // 4))
+        torch.cuda.empty_cache()
+        data = ws = None
 
     if not args.no_cpu:
         del data, ws

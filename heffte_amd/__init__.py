@@ -61,6 +61,17 @@ def _load():
         fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, ctypes.c_int]
         fn.restype = ctypes.c_int
     lib.heffte_mi355_reshape.argtypes = [ctypes.c_int, LL3, LL3, I3, I3, ctypes.c_int, vp, vp, vp]
+    lib.heffte_mi355_batch_create.argtypes = [ctypes.c_int, I3, LL3, LL3, ctypes.POINTER(plan_p)]
+    lib.heffte_mi355_batch_destroy.argtypes = [plan_p]
+    lib.heffte_mi355_batch_set_stream.argtypes = [plan_p, vp]
+    lib.heffte_mi355_batch_set_graph.argtypes = [plan_p, ctypes.c_int]
+    for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output"):
+        getattr(lib, f).argtypes = [plan_p]
+        getattr(lib, f).restype = ctypes.c_longlong
+    for f in ("forward_c2c", "forward_z2z", "backward_c2c", "backward_z2z"):
+        fn = getattr(lib, "heffte_mi355_batch_" + f)
+        fn.argtypes = [plan_p, vp, vp, ctypes.c_int]
+        fn.restype = ctypes.c_int
     lib.heffte_mi355_last_error.restype = ctypes.c_char_p
     lib.heffte_mi355_set_default_options.argtypes = [vp]
     return lib
@@ -77,6 +88,10 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_backward_c2c", "heffte_mi355_backward_z2z", "heffte_mi355_backward_c2s",
     "heffte_mi355_backward_z2d", "heffte_mi355_forward_s2s", "heffte_mi355_forward_d2d",
     "heffte_mi355_backward_s2s", "heffte_mi355_backward_d2d", "heffte_mi355_reshape", "heffte_mi355_last_error",
+    "heffte_mi355_batch_create", "heffte_mi355_batch_destroy", "heffte_mi355_batch_set_stream",
+    "heffte_mi355_batch_set_graph", "heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output",
+    "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
+    "heffte_mi355_batch_backward_z2z",
 )
 
 
@@ -281,3 +296,55 @@ def reshape(inbox, out_order, inarray, outarray, batch=1, stream=None):
     s = getattr(stream, "cuda_stream", stream)
     _check(lib.heffte_mi355_reshape(inarray.element_size(), il, ih, io, oo, batch, _ptr(inarray), _ptr(outarray),
                                     ctypes.c_void_p(s) if s else None), "reshape")
+
+
+class batch_plan:
+    """Independent c2c transforms of different sizes, each a box in order (0, 1, 2).
+
+    sizes: list of (nx, ny, nz); offsets in complex elements (None = packed back to back)."""
+
+    def __init__(self, sizes, in_offsets=None, out_offsets=None):
+        sizes = [tuple(int(v) for v in s) for s in sizes]
+        flat = [v for s in sizes for v in s]
+        S = (ctypes.c_int * len(flat))(*flat)
+        io = (ctypes.c_longlong * len(sizes))(*in_offsets) if in_offsets is not None else None
+        oo = (ctypes.c_longlong * len(sizes))(*out_offsets) if out_offsets is not None else None
+        h = ctypes.c_void_p()
+        _check(lib.heffte_mi355_batch_create(len(sizes), S, io, oo, ctypes.byref(h)), "batch_plan")
+        self._h = h
+        self.sizes = sizes
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and lib is not None:
+            lib.heffte_mi355_batch_destroy(h)
+            self._h = None
+
+    def size_input(self):
+        return lib.heffte_mi355_batch_size_input(self._h)
+
+    def size_output(self):
+        return lib.heffte_mi355_batch_size_output(self._h)
+
+    def set_stream(self, stream):
+        s = getattr(stream, "cuda_stream", stream)
+        _check(lib.heffte_mi355_batch_set_stream(self._h, ctypes.c_void_p(s) if s else None), "set_stream")
+
+    def set_graph(self, enable):
+        _check(lib.heffte_mi355_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
+
+    def _run(self, direction, inarray, outarray, scaling):
+        t = _dtype_name(inarray)
+        if t != _dtype_name(outarray) or t not in ("complex64", "complex128"):
+            raise heffte_input_error("batch transforms take complex64 or complex128 arrays of one type")
+        if inarray.numel() < self.size_input() or outarray.numel() < self.size_output():
+            raise heffte_input_error(f"{direction}() called with invalid array size")
+        key = "c2c" if t == "complex64" else "z2z"
+        fn = getattr(lib, f"heffte_mi355_batch_{direction}_{key}")
+        _check(fn(self._h, _ptr(inarray), _ptr(outarray), scaling), direction)
+
+    def forward(self, inarray, outarray, scaling=scale.none):
+        self._run("forward", inarray, outarray, scaling)
+
+    def backward(self, inarray, outarray, scaling=scale.none):
+        self._run("backward", inarray, outarray, scaling)

@@ -1,0 +1,65 @@
+// Batch of independent c2c 3D transforms of different sizes, run as three
+// persistent launches (one per axis) over a device work list, optionally
+// replayed from a captured hipGraph.  There is no counterpart in heFFTe, whose
+// batches share one geometry (include/heffte_fft3d.h:392-402).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "hmi_kernels.h"
+
+namespace hmi {
+
+class batch_plan {
+  public:
+    // sizes: 3 per transform (x fastest); offsets in complex elements (nullptr = packed)
+    batch_plan(int count, const int *sizes, const long long *in_off, const long long *out_off);
+    ~batch_plan();
+
+    void set_stream(hipStream_t s) { stream_ = s; }
+    void set_graph(bool on) { use_graph_ = on; }
+    int count() const { return (int)xf_.size(); }
+    long long input_elems() const { return in_extent_; }
+    long long output_elems() const { return out_extent_; }
+    // dir: -1 forward, +1 backward
+    void execute(int dir, bool dbl, const void *in, void *out, int scaling);
+
+  private:
+    struct device_state {
+        void *tw = nullptr;
+        batch_xform *xf = nullptr;
+        batch_item *items[3] = {nullptr, nullptr, nullptr};
+        int nitems[3] = {0, 0, 0};
+        generic_plan *plans = nullptr;
+        int *twoff = nullptr;
+        size_t lds = 0;
+        bool ready = false;
+    };
+    struct graph_key {
+        int dir, scaling;
+        const void *in;
+        void *out;
+        hipStream_t stream;
+        bool operator==(const graph_key &o) const {
+            return dir == o.dir && scaling == o.scaling && in == o.in && out == o.out && stream == o.stream;
+        }
+    };
+    void prepare(bool dbl);
+    void launch_phases(int dir, bool dbl, const void *in, void *out, int scaling, hipStream_t s);
+
+    std::vector<batch_xform> xf_;
+    std::vector<int> lengths_;  // distinct axis lengths, index = size_id
+    long long in_extent_ = 0, out_extent_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool use_graph_ = true;
+    device_state dev_[2];
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    graph_key gkey_[2] = {};
+    hipStream_t capture_stream_ = nullptr;
+    std::mutex mtx_;
+};
+
+}  // namespace hmi

@@ -9,11 +9,17 @@
 #include <string>
 
 #include "../../include/heffte_mi355.h"
+#include "hmi_batch.h"
 #include "hmi_plan.h"
 
 struct heffte_mi355_plan_s {
     uint32_t magic;
     hmi::fft_plan *impl;
+};
+
+struct heffte_mi355_batch_s {
+    uint32_t magic;
+    hmi::batch_plan *impl;
 };
 
 namespace {
@@ -195,6 +201,65 @@ int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const
         hmi::box3 b = make_box(low, high, in_order);
         hmi::permute(elem_bytes, b, out_order, batch, input, output, static_cast<hipStream_t>(hip_stream));
     });
+}
+
+static bool valid_batch(heffte_mi355_batch b) { return b && b->magic == kMagic && b->impl; }
+
+int heffte_mi355_batch_create(int count, int const *sizes, long long const *in_offsets, long long const *out_offsets,
+                              heffte_mi355_batch *batch) {
+    if (!batch) return fail(HMI_ERROR_INVALID_ARGUMENT, "null batch pointer");
+    *batch = nullptr;
+    return guarded([&] {
+        auto *impl = new hmi::batch_plan(count, sizes, in_offsets, out_offsets);
+        *batch = new heffte_mi355_batch_s{kMagic, impl};
+    });
+}
+
+int heffte_mi355_batch_destroy(heffte_mi355_batch batch) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    delete batch->impl;
+    batch->impl = nullptr;
+    batch->magic = 0;
+    delete batch;
+    return HMI_SUCCESS;
+}
+
+int heffte_mi355_batch_set_stream(heffte_mi355_batch batch, void *hip_stream) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    batch->impl->set_stream(static_cast<hipStream_t>(hip_stream));
+    return HMI_SUCCESS;
+}
+
+int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    batch->impl->set_graph(enable != 0);
+    return HMI_SUCCESS;
+}
+
+long long heffte_mi355_batch_size_input(heffte_mi355_batch batch) {
+    return valid_batch(batch) ? batch->impl->input_elems() : -1;
+}
+long long heffte_mi355_batch_size_output(heffte_mi355_batch batch) {
+    return valid_batch(batch) ? batch->impl->output_elems() : -1;
+}
+
+static int batch_run(heffte_mi355_batch batch, int dir, bool dbl, void const *in, void *out, int scale) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    if (scale < 0 || scale > 2) return fail(HMI_ERROR_INVALID_ARGUMENT, "scale must be 0, 1 or 2");
+    return guarded([&] { batch->impl->execute(dir, dbl, in, out, scale); });
+}
+
+int heffte_mi355_batch_forward_c2c(heffte_mi355_batch b, void const *in, void *out, int s) {
+    return batch_run(b, -1, false, in, out, s);
+}
+int heffte_mi355_batch_forward_z2z(heffte_mi355_batch b, void const *in, void *out, int s) {
+    return batch_run(b, -1, true, in, out, s);
+}
+int heffte_mi355_batch_backward_c2c(heffte_mi355_batch b, void const *in, void *out, int s) {
+    return batch_run(b, 1, false, in, out, s);
+}
+int heffte_mi355_batch_backward_z2z(heffte_mi355_batch b, void const *in, void *out, int s) {
+    return batch_run(b, 1, true, in, out, s);
 }
 
 char const *heffte_mi355_last_error(void) { return g_last_error.c_str(); }

@@ -27,6 +27,9 @@ const fast_kernel_info *find_fast_kernel(int n, bool dbl, bool wide);
 hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,
                           hipStream_t s);
 
+// One phase of a batch of mixed-size c2c transforms (see batch_args).
+hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
+
 // Permutation of a box between two layouts (heFFTe's in-node reshape).
 // a = source-fastest axis, b = destination-fastest axis (or any other axis
 // when both sides are fastest along a, same_fast = 1), c = remaining axis.

@@ -84,22 +84,25 @@ __device__ __forceinline__ void run_stages(C (&v)[E], C *lds, const C *__restric
     }
 }
 
+// Offset of line L (column-fastest numbering of column x b1 x batch).
+__device__ __forceinline__ long long line_offset(unsigned L, const pass_args &a, bool out) {
+    const unsigned q = L / (unsigned)a.ncol;
+    const unsigned c = L - q * (unsigned)a.ncol;
+    const unsigned b2 = q / (unsigned)a.nb1;
+    const unsigned b1 = q - b2 * (unsigned)a.nb1;
+    return out ? (long long)c * a.out_sc + (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2
+               : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
+}
+
+// One tile (T lines) of an n-point pass; the block has exactly (n / E) * T threads.
 template <typename R, typename RL, int E, int T, int PS>
-__global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
+__device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *lds) {
     using C = typename cvec<R>::type;
     constexpr int N = RL::n();
     constexpr int P = N / E;
     constexpr int RS = N + (N >> PS) + 1;
-    __shared__ C lds[T * RS];
 
     const int tid = threadIdx.x;
-    const int blk = blockIdx.x;
-    const int cb = blk % a.ncolblk;
-    const int rest = blk / a.ncolblk;
-    const int b1 = rest % a.nb1;
-    const int b2 = rest / a.nb1;
-    const int c0 = cb * T;
-
     int l, t, l2, t2;
     if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
     if (a.store_colfast) { l2 = tid % T; t2 = tid / T; } else { l2 = tid / P; t2 = tid % P; }
@@ -110,8 +113,9 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     const C *__restrict__ tw_half = tw + RL::twoff(RL::count);
     C v[E];
     {
-        const bool valid = (c0 + l) < a.ncol;
-        const long long base = (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2 + (long long)(c0 + l) * a.in_sc;
+        const unsigned L = tile * T + l;
+        const bool valid = L < (unsigned)a.nlines;
+        const long long base = valid ? line_offset(L, a, false) : 0;
         if (a.mode == kModeR2C) {
             // real line of 2N values read as N complex pairs z[m] = x[2m] + i x[2m+1]
             const R *__restrict__ src = static_cast<const R *>(a.in);
@@ -207,9 +211,10 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
         }
     }
 
-    const bool svalid = (c0 + l2) < a.ncol;
+    const unsigned L2 = tile * T + l2;
+    const bool svalid = L2 < (unsigned)a.nlines;
     const R s = (R)a.scale;
-    const long long obase = (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)(c0 + l2) * a.out_sc;
+    const long long obase = svalid ? line_offset(L2, a, true) : 0;
     if (a.mode == kModeR2C) {
         // Z = DFT_N(z); X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[N-k]) / 2, O = -i (Z[k] - conj Z[N-k]) / 2
 #pragma unroll
@@ -277,6 +282,14 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     }
 }
 
+template <typename R, typename RL, int E, int T, int PS>
+__global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
+    using C = typename cvec<R>::type;
+    constexpr int RS = RL::n() + (RL::n() >> PS) + 1;
+    __shared__ C lds[T * RS];
+    uniform_tile<R, RL, E, T, PS>(a, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------- generic pass
 
 template <int R, typename C>
@@ -293,32 +306,27 @@ __device__ __forceinline__ void generic_bfly(const C *sp, int istride, C *dp, in
     for (int q = 0; q < R; q++) dp[q * ostride] = x[q];
 }
 
+// One tile (g.lines lines) of an n-point pass with a runtime radix plan; any block size.
 template <typename R>
-__global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) {
+__device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned tile,
+                             typename cvec<R>::type *smem) {
     using C = typename cvec<R>::type;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     const int N = g.n, T = g.lines;
-    C *buf0 = reinterpret_cast<C *>(smem_raw);
+    C *buf0 = smem;
     C *buf1 = buf0 + (long long)T * N;
 
     const int tid = threadIdx.x, nth = blockDim.x;
-    const int blk = blockIdx.x;
-    const int cb = blk % a.ncolblk;
-    const int rest = blk / a.ncolblk;
-    const int b1 = rest % a.nb1;
-    const int b2 = rest / a.nb1;
-    const int c0 = cb * T;
     const R sg = a.conj ? R(-1) : R(1);
 
-    const long long ibase = (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2 + (long long)c0 * a.in_sc;
     for (int idx = tid; idx < T * N; idx += nth) {
         int l, j;
         if (a.load_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
         C x = cmk<C>(R(0), R(0));
-        if (c0 + l < a.ncol) {
-            const long long off = ibase + (long long)l * a.in_sc + (long long)j * a.in_sa;
+        const unsigned L = tile * T + l;
+        if (L < (unsigned)a.nlines) {
+            const long long lbase = line_offset(L, a, false);
+            const long long off = lbase + (long long)j * a.in_sa;
             const R *rsrc = static_cast<const R *>(a.in);
-            const long long lbase = ibase + (long long)l * a.in_sc;
             if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
                 const int jj = (j < (N + 1) / 2) ? 2 * j : 2 * (N - 1 - j) + 1;
                 x.x = rsrc[lbase + (long long)jj * a.in_sa];
@@ -332,7 +340,7 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
             } else if (a.in_real) {
                 x.x = static_cast<const R *>(a.in)[off];
             } else if (a.herm_load && j > N / 2) {
-                x = static_cast<const C *>(a.in)[ibase + (long long)l * a.in_sc + (long long)(N - j) * a.in_sa];
+                x = static_cast<const C *>(a.in)[lbase + (long long)(N - j) * a.in_sa];
                 x.y *= -sg;
             } else {
                 x = static_cast<const C *>(a.in)[off];
@@ -387,26 +395,90 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
     }
 
     const R sc = (R)a.scale;
-    const long long obase = (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2 + (long long)c0 * a.out_sc;
     for (int idx = tid; idx < T * N; idx += nth) {
         int l, j;
         if (a.store_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
-        if (c0 + l >= a.ncol || j >= a.store_len) continue;
+        const unsigned L = tile * T + l;
+        if (L >= (unsigned)a.nlines || j >= a.store_len) continue;
         const C y = src[l * N + j];
-        const long long off = obase + (long long)l * a.out_sc + (long long)j * a.out_sa;
+        const long long lobase = line_offset(L, a, true);
+        const long long off = lobase + (long long)j * a.out_sa;
         if (a.mode == kModeDCT2 || a.mode == kModeDST2) {
             const C w = static_cast<const C *>(a.tw)[g.postoff + j];
             const int k = (a.mode == kModeDST2) ? N - 1 - j : j;
-            static_cast<R *>(a.out)[obase + (long long)l * a.out_sc + (long long)k * a.out_sa] =
-                R(2) * (w.x * y.x - w.y * y.y) * sc;
+            static_cast<R *>(a.out)[lobase + (long long)k * a.out_sa] = R(2) * (w.x * y.x - w.y * y.y) * sc;
         } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
             const int jj = (j < (N + 1) / 2) ? 2 * j : 2 * (N - 1 - j) + 1;
             R r = R(2) * y.x;
             if (a.mode == kModeDST3 && (jj & 1)) r = -r;
-            static_cast<R *>(a.out)[obase + (long long)l * a.out_sc + (long long)jj * a.out_sa] = r * sc;
+            static_cast<R *>(a.out)[lobase + (long long)jj * a.out_sa] = r * sc;
         } else if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
         else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
     }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    generic_tile<R>(a, g, blockIdx.x, reinterpret_cast<typename cvec<R>::type *>(smem_raw));
+}
+
+// --------------------------------------------------------------- batched
+
+// One phase (one axis) of a batch of independent c2c transforms of different
+// sizes.  A persistent grid walks the work list; each item is one tile of
+// lines of one transform, run by the runtime-plan tile of its length.
+template <typename R>
+__global__ void __launch_bounds__(256) fft_batched(batch_args b) {
+    using C = typename cvec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *smem = reinterpret_cast<C *>(smem_raw);
+    const int ax = b.axis;
+    const int col = (ax == 0) ? 1 : 0;
+    const int b1 = 3 - ax - col;
+    for (int i = blockIdx.x; i < b.nitems; i += gridDim.x) {
+        const batch_item it = b.items[i];
+        const batch_xform x = b.xforms[it.xform];
+        const long long st[3] = {1, x.n[0], (long long)x.n[0] * x.n[1]};
+        pass_args a;
+        a.in = b.from_in ? static_cast<const void *>(static_cast<const C *>(b.in) + x.in_off)
+                         : static_cast<const void *>(static_cast<const C *>(b.out) + x.out_off);
+        a.out = static_cast<C *>(b.out) + x.out_off;
+        a.tw = static_cast<const C *>(b.tw) + b.twoff[x.size_id[ax]];
+        a.in_sa = a.out_sa = st[ax];
+        a.in_sc = a.out_sc = st[col];
+        a.in_sb1 = a.out_sb1 = st[b1];
+        a.in_sb2 = a.out_sb2 = 0;
+        a.ncol = x.n[col];
+        a.nb1 = x.n[b1];
+        a.nbatch = 1;
+        a.nlines = x.n[col] * x.n[b1];
+        // the x axis is contiguous: the y and z phases walk it with consecutive threads
+        a.load_colfast = a.store_colfast = (ax != 0 && x.n[0] > 1) ? 1 : 0;
+        a.conj = b.conj;
+        a.in_real = a.out_real = 0;
+        a.store_len = x.n[ax];
+        a.herm_load = 0;
+        a.mode = kModeC2C;
+        const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
+        a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
+        generic_tile<R>(a, b.plans[x.size_id[ax]], it.tile, smem);
+        __syncthreads();
+    }
+}
+
+hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s) {
+    static bool configured = false;
+    if (!configured) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        configured = true;
+    }
+    if (dbl) fft_batched<double><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(b);
+    else fft_batched<float><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(b);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------- permutation

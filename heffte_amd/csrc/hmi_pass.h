@@ -38,7 +38,7 @@ struct pass_args {
     int ncol;      // number of columns
     int nb1;       // extent of the third axis
     int nbatch;    // number of transforms in the batch
-    int ncolblk;   // ceil(ncol / lines per block)
+    int nlines;    // ncol * nb1 * nbatch: lines are numbered column-fastest, then b1, then batch
     int load_colfast;   // 1: consecutive threads walk the column axis on load
     int store_colfast;  // 1: ... on store
     int conj;      // 1: backward transform (conjugate on load and on store)
@@ -59,6 +59,34 @@ struct generic_plan {
     int twoff[kMaxStages];     // offset (complex units) of stage twiddles w_{Ls R}^{p k}
     int rootoff[kMaxStages];   // offset of w_R^j, j < R (used by radices without a hard-coded butterfly)
     int postoff;               // offset of the real-data twiddles (modes other than kModeC2C)
+};
+
+// Batched mixed-size transforms (one phase = one axis of every transform).
+// Every transform is a c2c box in order (0, 1, 2) at its own element offset.
+struct batch_xform {
+    long long in_off, out_off;  // element offsets from the batch's input / output base
+    int n[3];
+    int size_id[3];             // index of the radix plan of each axis length
+};
+
+struct batch_item {
+    int xform;  // transform
+    int tile;   // tile of lines along the phase axis
+};
+
+struct batch_args {
+    const void *in;
+    void *out;
+    const void *tw;               // concatenated twiddle tables of every length
+    const batch_xform *xforms;
+    const batch_item *items;
+    const generic_plan *plans;    // one per distinct length
+    const int *twoff;             // table offset (complex units) of each length
+    int nitems;
+    int axis;       // 0, 1 or 2
+    int from_in;    // 1: read the input arrays (first phase), 0: read the output arrays in place
+    int conj;       // backward transform
+    int scaling;    // 0 none, 1 full (1/N of each transform), 2 symmetric; applied by the last phase
 };
 
 }  // namespace hmi

@@ -389,6 +389,35 @@ std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int ta
 
 }  // namespace
 
+generic_plan make_generic_plan(int n, int lines) {
+    generic_plan g{};
+    g.n = n;
+    std::vector<int> rad = factorize(n);
+    if ((int)rad.size() > kMaxStages) throw std::runtime_error("too many radix stages");
+    g.nstages = (int)rad.size();
+    int off = 0, ls = 1;
+    for (int s = 0; s < g.nstages; s++) {
+        g.radix[s] = rad[s];
+        g.twoff[s] = off;
+        off += ls * (rad[s] - 1);
+        ls *= rad[s];
+    }
+    for (int s = 0; s < g.nstages; s++) {
+        g.rootoff[s] = off;
+        if (!has_butterfly(rad[s])) off += rad[s];
+    }
+    g.postoff = off;
+    g.lines = lines;
+    return g;
+}
+
+std::vector<float> generic_table_f32(const generic_plan &g) {
+    return make_table<float>(g.radix, g.nstages, true, 0, 1);
+}
+std::vector<double> generic_table_f64(const generic_plan &g) {
+    return make_table<double>(g.radix, g.nstages, true, 0, 1);
+}
+
 const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     const int n = p.n;
     const long long key = ((long long)n << 8) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
@@ -473,7 +502,8 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.ncol = (int)p.lout.n[p.col];
         a.nb1 = (int)p.lout.n[p.b1];
         a.nbatch = batch;
-        a.ncolblk = (a.ncol + p.lines - 1) / p.lines;
+        if ((long long)a.ncol * a.nb1 * batch > 0x7fffffffLL) throw std::runtime_error("too many lines in one pass");
+        a.nlines = a.ncol * a.nb1 * batch;
         a.load_colfast = p.load_colfast;
         a.store_colfast = p.store_colfast;
         // r2r modes run their inverse through explicit conjugation inside the kernel
@@ -484,8 +514,7 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.herm_load = p.herm_load;
         a.mode = p.mode;
         a.scale = p.last ? scale : 1.0;
-        const long long nblocks = (long long)a.ncolblk * a.nb1 * batch;
-        if (nblocks > 0x7fffffffLL) throw std::runtime_error("grid too large");
+        const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
         if (p.fast) {
             check_hip(p.fast->launch(a, nblocks, stream_), "fast FFT pass");
         } else {

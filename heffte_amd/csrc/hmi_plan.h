@@ -113,6 +113,12 @@ class fft_plan {
     std::map<long long, twiddles> tw_;
 };
 
+// Runtime radix plan of length n for the generic tile (stage/root/post offsets
+// filled in; lines = lines per tile) and its twiddle table in either precision.
+generic_plan make_generic_plan(int n, int lines);
+std::vector<float> generic_table_f32(const generic_plan &g);
+std::vector<double> generic_table_f64(const generic_plan &g);
+
 // Permutation of a box between two orders (reshape3d_transpose semantics).
 void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,
              hipStream_t s);

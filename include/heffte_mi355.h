@@ -141,6 +141,28 @@ int heffte_mi355_backward_d2d(heffte_mi355_plan plan, int batch, double const *i
 int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const high[3], int const in_order[3],
                          int const out_order[3], int batch, void const *input, void *output, void *hip_stream);
 
+/* ---- Batches of independent transforms of different sizes (no heFFTe counterpart:
+ * heFFTe batches share one geometry, include/heffte_fft3d.h:392-402).  Each transform is a
+ * c2c box of sizes[3*i .. 3*i+2] in order (0, 1, 2) stored at element offset in_offsets[i]
+ * of the input array and out_offsets[i] of the output array (NULL = packed back to back;
+ * out_offsets NULL = same as in_offsets).  The forward/backward calls run three persistent
+ * launches (one per axis) over a device work list, replayed from a captured hipGraph unless
+ * heffte_mi355_batch_set_graph(batch, 0) was called.  scale applies per transform (1/N_i). */
+typedef struct heffte_mi355_batch_s *heffte_mi355_batch;
+
+int heffte_mi355_batch_create(int count, int const *sizes, long long const *in_offsets,
+                              long long const *out_offsets, heffte_mi355_batch *batch);
+int heffte_mi355_batch_destroy(heffte_mi355_batch batch);
+int heffte_mi355_batch_set_stream(heffte_mi355_batch batch, void *hip_stream);
+int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
+/* Elements (complex) the input / output arrays must hold. */
+long long heffte_mi355_batch_size_input(heffte_mi355_batch batch);
+long long heffte_mi355_batch_size_output(heffte_mi355_batch batch);
+int heffte_mi355_batch_forward_c2c(heffte_mi355_batch batch, void const *input, void *output, int scale);
+int heffte_mi355_batch_forward_z2z(heffte_mi355_batch batch, void const *input, void *output, int scale);
+int heffte_mi355_batch_backward_c2c(heffte_mi355_batch batch, void const *input, void *output, int scale);
+int heffte_mi355_batch_backward_z2z(heffte_mi355_batch batch, void const *input, void *output, int scale);
+
 /* Text of the last error raised on the calling thread ("" if none). */
 char const *heffte_mi355_last_error(void);
 

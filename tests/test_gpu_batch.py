@@ -1,0 +1,123 @@
+"""GPU parity of the batched mixed-size transforms (persistent work-list kernel + hipGraph)
+against the FP64 oracle, transform by transform.  Sizes are drawn from the smooth lengths
+{16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64, 80, 96, 105, 112, 128} (radices 2, 3, 5, 7).
+The captured graph is replayed several times with eager work in between and every replay
+is checked."""
+import numpy as np
+import pytest
+
+import fftcase as fc
+
+pytestmark = pytest.mark.gpu
+
+SMOOTH = [16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64, 80, 96, 105, 112, 128]
+TOL = {"complex64": 5e-6, "complex128": 1e-13}
+
+
+def draw_sizes(count, seed, choices):
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice(choices, 3)) for _ in range(count)]
+
+
+def oracle_each(oracle, sizes, offsets, x, sign, scaling):
+    out = []
+    for s, off in zip(sizes, offsets):
+        n = int(np.prod(s))
+        b = oracle.Box.make((0, 0, 0), [v - 1 for v in s])
+        sc = 1.0 if scaling == 0 else (1.0 / n if scaling == 1 else 1.0 / np.sqrt(n))
+        out.append(oracle.fft3d_c2c(b, b, x[off:off + n], sign, sc))
+    return out
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_vs_oracle(hmi, cuda, oracle, dtype, graph):
+    import torch
+    sizes = draw_sizes(30, 4, [16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64])
+    plan = hmi.batch_plan(sizes)
+    plan.set_graph(graph)
+    n = plan.size_input()
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in sizes])[:-1]
+    x = fc.uniform_complex(n, 4)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    tx = torch.from_numpy(x).to(cuda).to(tdt)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    y = fc.to_host(ty)
+    refs = oracle_each(oracle, sizes, offsets, x, -1, 0)
+    for s, off, r in zip(sizes, offsets, refs):
+        assert fc.rel_l2(y[off:off + r.size], r) <= TOL[dtype], s
+    # backward with full scaling, in place, returns the input
+    plan.backward(ty, ty, hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(ty), x) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_batch_all_smooth_lengths_and_offsets(hmi, cuda, oracle, dtype):
+    """Every smooth length on every axis, output at shuffled, gapped offsets, symmetric scale."""
+    import torch
+    sizes = [(SMOOTH[i], SMOOTH[(i + 5) % 17], SMOOTH[(i + 11) % 17]) for i in range(17)]
+    counts = [int(np.prod(s)) for s in sizes]
+    in_off = list(np.cumsum([0] + counts)[:-1])
+    order = np.random.default_rng(1).permutation(len(sizes))
+    out_off = [0] * len(sizes)
+    pos = 7  # leave gaps between the outputs
+    for i in order:
+        out_off[i] = pos
+        pos += counts[i] + 3
+    plan = hmi.batch_plan(sizes, in_off, out_off)
+    x = fc.uniform_complex(plan.size_input(), 9)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    tx = torch.from_numpy(x).to(cuda).to(tdt)
+    ty = torch.zeros(plan.size_output(), dtype=tdt, device=cuda)
+    plan.forward(tx, ty, hmi.scale.symmetric)
+    y = fc.to_host(ty)
+    refs = oracle_each(oracle, sizes, in_off, x, -1, 2)
+    for s, off, r in zip(sizes, out_off, refs):
+        assert fc.rel_l2(y[off:off + r.size], r) <= TOL[dtype], s
+    # the gaps were not written
+    mask = np.ones(y.size, bool)
+    for off, c in zip(out_off, counts):
+        mask[off:off + c] = False
+    assert np.all(y[mask] == 0)
+
+
+def test_graph_replays(hmi, cuda, oracle):
+    """Replays of the captured graph, with eager work and synchronisations between them."""
+    import torch
+    sizes = draw_sizes(12, 7, [16, 24, 32, 48, 64])
+    plan = hmi.batch_plan(sizes)
+    x = fc.uniform_complex(plan.size_input(), 2)
+    tx = torch.from_numpy(x).to(cuda).to(torch.complex128)
+    ty = torch.empty_like(tx)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in sizes])[:-1]
+    refs = oracle_each(oracle, sizes, offsets, x, -1, 1)
+    for rep in range(4):
+        ty.fill_(float(rep))
+        junk = torch.randn(1 << 16, device=cuda).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        plan.forward(tx, ty, hmi.scale.full)
+        y = fc.to_host(ty)
+        for s, off, r in zip(sizes, offsets, refs):
+            assert fc.rel_l2(y[off:off + r.size], r) <= 1e-13, (rep, s)
+        del junk
+
+
+def test_batch_matches_single_plans(hmi, cuda):
+    """The batched result equals, to rounding, the per-transform fft3d plans."""
+    import torch
+    sizes = draw_sizes(8, 3, SMOOTH)
+    plan = hmi.batch_plan(sizes)
+    x = fc.uniform_complex(plan.size_input(), 5)
+    tx = torch.from_numpy(x).to(cuda).to(torch.complex64)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    off = 0
+    for s in sizes:
+        n = int(np.prod(s))
+        b = hmi.box3d([0, 0, 0], [v - 1 for v in s])
+        p = hmi.fft3d(hmi.backend.mi355, b, b)
+        out = torch.empty(n, dtype=tx.dtype, device=cuda)
+        p.forward(tx[off:off + n], out)
+        assert fc.rel_l2(fc.to_host(ty[off:off + n]), fc.to_host(out).astype(np.complex128)) <= 5e-6
+        off += n

@@ -68,6 +68,7 @@ batch_plan::~batch_plan() {
         for (auto *p : d.items) (void)hipFree(p);
         (void)hipFree(d.plans);
         (void)hipFree(d.twoff);
+        (void)hipFree(d.kid);
     }
 }
 
@@ -77,25 +78,37 @@ void batch_plan::prepare(bool dbl) {
     const size_t elem = dbl ? 16 : 8;
     // one radix plan and twiddle table per distinct length; lines per tile bounded by 64 KiB of LDS
     std::vector<generic_plan> plans;
-    std::vector<int> twoff;
+    std::vector<int> twoff, kids;
     std::vector<float> tf;
     std::vector<double> td;
     size_t lds = 0;
     for (int n : lengths_) {
-        const size_t per_line = 2 * (size_t)n * elem;
-        int lines = (int)std::min<size_t>(64, std::max<size_t>(1, (64 * 1024) / per_line));
-        generic_plan g = make_generic_plan(n, lines);
-        lds = std::max(lds, per_line * lines);
+        batch_kernel_info info{};
+        const int kid = find_batch_kernel(n, &info);
+        generic_plan g{};
+        if (kid >= 0) {
+            // compile-time tile: only its line count is read from the plan
+            g.n = n;
+            g.lines = dbl ? info.lines_f64 : info.lines_f32;
+            lds = std::max(lds, dbl ? info.lds_f64 : info.lds_f32);
+        } else {
+            const size_t per_line = 2 * (size_t)n * elem;
+            // keep every tile of the batch within 40 KiB of LDS: several resident blocks per CU
+            const int lines = (int)std::min<size_t>(64, std::max<size_t>(1, (40 * 1024) / per_line));
+            g = make_generic_plan(n, lines);
+            lds = std::max(lds, per_line * lines);
+        }
         if (dbl) {
             twoff.push_back((int)(td.size() / 2));
-            auto t = generic_table_f64(g);
+            auto t = kid >= 0 ? stage_table_f64(info.radix, info.nstages) : generic_table_f64(g);
             td.insert(td.end(), t.begin(), t.end());
         } else {
             twoff.push_back((int)(tf.size() / 2));
-            auto t = generic_table_f32(g);
+            auto t = kid >= 0 ? stage_table_f32(info.radix, info.nstages) : generic_table_f32(g);
             tf.insert(tf.end(), t.begin(), t.end());
         }
         plans.push_back(g);
+        kids.push_back(kid);
     }
     // work lists: every tile of every transform, one list per axis
     for (int ax = 0; ax < 3; ax++) {
@@ -114,6 +127,7 @@ void batch_plan::prepare(bool dbl) {
     }
     d.plans = upload(plans);
     d.twoff = upload(twoff);
+    d.kid = upload(kids);
     d.xf = upload(xf_);
     if (dbl) d.tw = upload(td);
     else d.tw = upload(tf);
@@ -132,6 +146,7 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
         b.items = d.items[ax];
         b.plans = d.plans;
         b.twoff = d.twoff;
+        b.kid = d.kid;
         b.nitems = d.nitems[ax];
         b.axis = ax;
         b.from_in = (ax == 0) ? 1 : 0;

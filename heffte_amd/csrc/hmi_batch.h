@@ -35,6 +35,7 @@ class batch_plan {
         int nitems[3] = {0, 0, 0};
         generic_plan *plans = nullptr;
         int *twoff = nullptr;
+        int *kid = nullptr;
         size_t lds = 0;
         bool ready = false;
     };

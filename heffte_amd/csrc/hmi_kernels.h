@@ -27,6 +27,17 @@ const fast_kernel_info *find_fast_kernel(int n, bool dbl, bool wide);
 hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,
                           hipStream_t s);
 
+// Compile-time tile of the batched kernel for one length: lines per tile and LDS bytes
+// for each precision, and the complex-unit size of its stage twiddle table.
+struct batch_kernel_info {
+    int n, lines_f32, lines_f64;
+    int radix[8];
+    int nstages;
+    size_t lds_f32, lds_f64;
+};
+// Returns the tile id for length n (or -1: the runtime-plan tile runs it).
+int find_batch_kernel(int n, batch_kernel_info *info);
+
 // One phase of a batch of mixed-size c2c transforms (see batch_args).
 hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
 

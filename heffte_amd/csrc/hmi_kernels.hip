@@ -106,6 +106,10 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     int l, t, l2, t2;
     if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
     if (a.store_colfast) { l2 = tid % T; t2 = tid / T; } else { l2 = tid / P; t2 = tid % P; }
+    // a block may carry more threads than P * T (batched kernel): the extra threads take
+    // part in the barriers and work on a scratch LDS row T that the caller provides
+    const bool active = tid < P * T;
+    if (!active) { l = l2 = T; t = t2 = 0; }
 
     const R sg = a.conj ? R(-1) : R(1);
     const C *__restrict__ tw = static_cast<const C *>(a.tw);
@@ -114,7 +118,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     C v[E];
     {
         const unsigned L = tile * T + l;
-        const bool valid = L < (unsigned)a.nlines;
+        const bool valid = active && L < (unsigned)a.nlines;
         const long long base = valid ? line_offset(L, a, false) : 0;
         if (a.mode == kModeR2C) {
             // real line of 2N values read as N complex pairs z[m] = x[2m] + i x[2m+1]
@@ -212,7 +216,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     }
 
     const unsigned L2 = tile * T + l2;
-    const bool svalid = L2 < (unsigned)a.nlines;
+    const bool svalid = active && L2 < (unsigned)a.nlines;
     const R s = (R)a.scale;
     const long long obase = svalid ? line_offset(L2, a, true) : 0;
     if (a.mode == kModeR2C) {
@@ -428,8 +432,55 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
 // One phase (one axis) of a batch of independent c2c transforms of different
 // sizes.  A persistent grid walks the work list; each item is one tile of
 // lines of one transform, run by the runtime-plan tile of its length.
+// Compile-time tiles of the batched kernel: (n, E, lines T for complex<float>, pad shift,
+// radices).  P = n / E threads per line; P * T <= 256 (complex<float>, 256-thread blocks) and
+// P * (T / 2) <= 128 (complex<double>, 128-thread blocks); LDS = (T + 1) rows stays <= 40 KiB.
+#define HMI_BATCH_TABLE(X)            \
+    X(0, 16, 16, 256, 4, 16)          \
+    X(1, 20, 10, 128, 4, 2, 2, 5)     \
+    X(2, 24, 12, 128, 4, 3, 4, 2)     \
+    X(3, 27, 9, 84, 4, 3, 3, 3)       \
+    X(4, 28, 14, 128, 4, 2, 2, 7)     \
+    X(5, 32, 8, 64, 3, 4, 8)          \
+    X(6, 40, 10, 64, 4, 2, 2, 2, 5)   \
+    X(7, 45, 15, 84, 4, 3, 3, 5)      \
+    X(8, 48, 12, 64, 4, 4, 4, 3)      \
+    X(9, 56, 14, 64, 4, 2, 2, 2, 7)   \
+    X(10, 63, 21, 64, 4, 3, 3, 7)     \
+    X(11, 64, 8, 32, 3, 8, 8)         \
+    X(12, 80, 10, 32, 4, 2, 2, 2, 2, 5) \
+    X(13, 96, 12, 32, 4, 4, 4, 2, 3)  \
+    X(14, 112, 14, 32, 4, 2, 2, 2, 2, 7) \
+    X(15, 128, 16, 32, 4, 16, 8)
+
+#define HMI_BATCH_INFO(K, N, E, T, PS, ...) {N, T, T / 2, {__VA_ARGS__}, radices<__VA_ARGS__>::count, \
+    (size_t)(T + 1) * (N + (N >> PS) + 1) * 8, (size_t)(T / 2 + 1) * (N + (N >> PS) + 1) * 16},
+static const batch_kernel_info kBatch[] = {HMI_BATCH_TABLE(HMI_BATCH_INFO)};
+
+int find_batch_kernel(int n, batch_kernel_info *info) {
+    for (int k = 0; k < (int)(sizeof(kBatch) / sizeof(kBatch[0])); k++)
+        if (kBatch[k].n == n) {
+            if (info) *info = kBatch[k];
+            return k;
+        }
+    return -1;
+}
+
 template <typename R>
-__global__ void __launch_bounds__(256) fft_batched(batch_args b) {
+__device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const generic_plan &g, unsigned tile,
+                                           typename cvec<R>::type *smem) {
+    constexpr bool D = sizeof(R) == 8;
+#define HMI_BATCH_CASE(K, N, E, T, PS, ...) \
+    case K: uniform_tile<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem); break;
+    switch (kid) {
+        HMI_BATCH_TABLE(HMI_BATCH_CASE)
+        default: generic_tile<R>(a, g, tile, smem);
+    }
+#undef HMI_BATCH_CASE
+}
+
+template <typename R>
+__global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256) fft_batched(batch_args b) {
     using C = typename cvec<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *smem = reinterpret_cast<C *>(smem_raw);
@@ -462,7 +513,7 @@ __global__ void __launch_bounds__(256) fft_batched(batch_args b) {
         a.mode = kModeC2C;
         const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
         a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
-        generic_tile<R>(a, b.plans[x.size_id[ax]], it.tile, smem);
+        batch_tile<R>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);
         __syncthreads();
     }
 }
@@ -476,7 +527,7 @@ hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
         configured = true;
     }
-    if (dbl) fft_batched<double><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(b);
+    if (dbl) fft_batched<double><<<dim3((unsigned)nblocks), dim3(128), lds_bytes, s>>>(b);
     else fft_batched<float><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(b);
     return hipGetLastError();
 }

@@ -82,6 +82,7 @@ struct batch_args {
     const batch_item *items;
     const generic_plan *plans;    // one per distinct length
     const int *twoff;             // table offset (complex units) of each length
+    const int *kid;               // compile-time tile of each length (-1: runtime radix plan)
     int nitems;
     int axis;       // 0, 1 or 2
     int from_in;    // 1: read the input arrays (first phase), 0: read the output arrays in place

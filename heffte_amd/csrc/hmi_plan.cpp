@@ -417,6 +417,12 @@ std::vector<float> generic_table_f32(const generic_plan &g) {
 std::vector<double> generic_table_f64(const generic_plan &g) {
     return make_table<double>(g.radix, g.nstages, true, 0, 1);
 }
+std::vector<float> stage_table_f32(const int *radix, int nstages) {
+    return make_table<float>(radix, nstages, false, 0, 1);
+}
+std::vector<double> stage_table_f64(const int *radix, int nstages) {
+    return make_table<double>(radix, nstages, false, 0, 1);
+}
 
 const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     const int n = p.n;

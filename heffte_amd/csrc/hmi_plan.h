@@ -118,6 +118,9 @@ class fft_plan {
 generic_plan make_generic_plan(int n, int lines);
 std::vector<float> generic_table_f32(const generic_plan &g);
 std::vector<double> generic_table_f64(const generic_plan &g);
+// Stage twiddles of a compile-time radix plan (no root or real-data tables).
+std::vector<float> stage_table_f32(const int *radix, int nstages);
+std::vector<double> stage_table_f64(const int *radix, int nstages);
 
 // Permutation of a box between two orders (reshape3d_transpose semantics).
 void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,

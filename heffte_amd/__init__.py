@@ -55,6 +55,7 @@ def _load():
         getattr(lib, f).restype = ctypes.c_longlong
     lib.heffte_mi355_get_kind.argtypes = [plan_p]
     lib.heffte_mi355_set_stream.argtypes = [plan_p, vp]
+    lib.heffte_mi355_set_precision.argtypes = [plan_p, ctypes.c_int]
     for f in ("forward_c2c", "forward_z2z", "forward_s2c", "forward_d2z", "backward_c2c", "backward_z2z",
               "backward_c2s", "backward_z2d", "forward_s2s", "forward_d2d", "backward_s2s", "backward_d2d"):
         fn = getattr(lib, "heffte_mi355_" + f)
@@ -91,7 +92,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_create", "heffte_mi355_batch_destroy", "heffte_mi355_batch_set_stream",
     "heffte_mi355_batch_set_graph", "heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output",
     "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
-    "heffte_mi355_batch_backward_z2z",
+    "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision",
 )
 
 
@@ -105,6 +106,13 @@ class scale:
     none = 0
     full = 1
     symmetric = 2
+
+
+class precision:
+    """Precision form of the complex<float> 512-point passes (HMI_PRECISION_*)."""
+    fp32 = 0
+    split_fp16 = 1
+    split_bf16 = 2
 
 
 class kind:
@@ -194,6 +202,10 @@ class fft_plan:
 
     def size_workspace(self):
         return lib.heffte_mi355_size_workspace(self._h)
+
+    def set_precision(self, form):
+        """precision.fp32 (default), precision.split_fp16 or precision.split_bf16."""
+        _check(lib.heffte_mi355_set_precision(self._h, int(form)), "set_precision")
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream, a raw hipStream_t address, or None (default stream)."""

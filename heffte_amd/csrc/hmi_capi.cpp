@@ -157,6 +157,14 @@ int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream) {
     return HMI_SUCCESS;
 }
 
+int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    if (precision < HMI_PRECISION_FP32 || precision > HMI_PRECISION_SPLIT_BF16)
+        return fail(HMI_ERROR_INVALID_ARGUMENT, "unknown precision form");
+    plan->impl->set_precision(precision);
+    return HMI_SUCCESS;
+}
+
 int heffte_mi355_forward_c2c(heffte_mi355_plan p, int b, void const *in, void *out, void *ws, int s) {
     return run_checked(p, false, -1, false, false, false, b, in, out, ws, s);
 }

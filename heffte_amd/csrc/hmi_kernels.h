@@ -41,6 +41,12 @@ int find_batch_kernel(int n, batch_kernel_info *info);
 // One phase of a batch of mixed-size c2c transforms (see batch_args).
 hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
 
+// Precision forms of the c2c passes: plain FP32/FP64 vector ALU, or the 512-point
+// complex<float> pass on the matrix cores with split-precision operands.
+enum : int { kPrecPlain = 0, kSplitF16 = 1, kSplitBF16 = 2 };
+// 512-point complex<float> c2c pass on MFMA (16 lines per block); tw = W32, W16, W512 tables.
+hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s);
+
 // Permutation of a box between two layouts (heFFTe's in-node reshape).
 // a = source-fastest axis, b = destination-fastest axis (or any other axis
 // when both sides are fastest along a, same_fast = 1), c = remaining axis.

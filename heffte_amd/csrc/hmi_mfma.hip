@@ -1,0 +1,252 @@
+// 512-point c2c pass of complex<float> data computed on the matrix cores (gfx950 MFMA).
+//
+// n = 512 = 32 x 16 (four-step): with n = 16 n1 + n2 and k = k1 + 32 k2,
+//   Y[k1][n2] = sum_n1 W32^{n1 k1} x[16 n1 + n2]            (step A, 32-point DFTs)
+//   X[k1 + 32 k2] = sum_n2 W16^{n2 k2} (W512^{n2 k1} Y[k1][n2])   (twiddle, step C, 16-point DFTs)
+// Steps A and C are real GEMMs on the block form [Re F, -Im F; Im F, Re F] x [Re v; Im v]
+// with v_mfma_f32_32x32x16_{f16,bf16} (FP32 accumulation).  Accuracy comes from splitting
+// every FP32 operand into low-precision terms:
+//   kSplitF16:  x = hi + lo (two FP16 terms), products hi*hi + hi*lo + lo*hi
+//   kSplitBF16: x = hi + mid + lo (three BF16 terms), products hh + hm + mh + hl + lh + mm
+// and from a power-of-two scale per line (pencil) that puts the line maximum just below 1 so
+// both terms stay in the normal range of the 16-bit format; the scale is undone on store.
+//
+// MFMA 32x32x16 lane layout (gfx950): A[i][k] in lane i + 32 (k / 8), element k % 8;
+// B[k][j] in lane j + 32 (k / 8), element k % 8; C[i][j] in lane j + 32 ((i / 4) % 2),
+// element (i % 4) + 4 (i / 8).
+#include <hip/hip_runtime.h>
+
+#include "hmi_device.h"
+#include "hmi_kernels.h"
+
+namespace hmi {
+
+namespace {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+template <int PREC> struct split_traits;
+
+template <> struct split_traits<kSplitF16> {
+    static constexpr int terms = 2;
+    using vec = half8_t;
+    using scalar = _Float16;
+    __device__ static f32x16_t mma(vec a, vec b, f32x16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct split_traits<kSplitBF16> {
+    static constexpr int terms = 3;
+    using vec = bf8_t;
+    using scalar = __bf16;
+    __device__ static f32x16_t mma(vec a, vec b, f32x16_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// split 8 FP32 values into TERMS low-precision vectors
+template <int PREC>
+__device__ __forceinline__ void split8(const float (&x)[8], typename split_traits<PREC>::vec (&out)[3]) {
+    using S = split_traits<PREC>;
+    using scal = typename S::scalar;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        float r = x[e];
+#pragma unroll
+        for (int t = 0; t < S::terms; t++) {
+            const scal p = (scal)r;
+            out[t][e] = p;
+            r -= (float)p;
+        }
+    }
+}
+
+// acc += A * B through the split products
+template <int PREC>
+__device__ __forceinline__ f32x16_t split_mma(const typename split_traits<PREC>::vec (&a)[3],
+                                              const typename split_traits<PREC>::vec (&b)[3], f32x16_t acc) {
+    using S = split_traits<PREC>;
+    if constexpr (S::terms == 2) {
+        acc = S::mma(a[0], b[0], acc);
+        acc = S::mma(a[0], b[1], acc);
+        acc = S::mma(a[1], b[0], acc);
+    } else {
+        acc = S::mma(a[0], b[0], acc);
+        acc = S::mma(a[0], b[1], acc);
+        acc = S::mma(a[1], b[0], acc);
+        acc = S::mma(a[0], b[2], acc);
+        acc = S::mma(a[2], b[0], acc);
+        acc = S::mma(a[1], b[1], acc);
+    }
+    return acc;
+}
+
+__device__ __forceinline__ int padj3(int j) { return j + (j >> 3); }
+
+__device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bool out) {
+    const unsigned q = L / (unsigned)a.ncol;
+    const unsigned c = L - q * (unsigned)a.ncol;
+    const unsigned b2 = q / (unsigned)a.nb1;
+    const unsigned b1 = q - b2 * (unsigned)a.nb1;
+    return out ? (long long)c * a.out_sc + (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2
+               : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
+}
+
+}  // namespace
+
+// One block = 16 lines x 512 points, 1024 threads (16 waves).  The twiddle table holds
+// W32^r (32 entries), W16^r (16) and W512^r (512), complex<float>, rounded from long double.
+template <int PREC>
+__global__ void __launch_bounds__(1024) fft512_mfma(pass_args a) {
+    constexpr int N = 512, T = 16, P = 64, E = 8;
+    constexpr int RS = N + (N >> 3) + 1;
+    __shared__ float2 lds[T * RS];
+    __shared__ float lscale[T];
+    using vec = typename split_traits<PREC>::vec;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const unsigned tile = blockIdx.x;
+    const float sg = a.conj ? -1.0f : 1.0f;
+    const float2 *__restrict__ w32 = static_cast<const float2 *>(a.tw);
+    const float2 *__restrict__ w16 = w32 + 32;
+    const float2 *__restrict__ w512 = w32 + 48;
+    float *ldsf = reinterpret_cast<float *>(lds);
+
+    // ---- load the tile (coalesced along the axis or along the lines), conjugate for backward
+    {
+        int l, t;
+        if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
+        const unsigned L = tile * T + l;
+        const bool valid = L < (unsigned)a.nlines;
+        const long long base = valid ? line_off(L, a, false) : 0;
+        const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int pos = t + P * e;
+            float2 x = valid ? src[base + (long long)pos * a.in_sa] : make_float2(0.f, 0.f);
+            x.y *= sg;
+            lds[l * RS + padj3(pos)] = x;
+        }
+    }
+    __syncthreads();
+    // ---- per-line power-of-two scale: wave w reduces line w
+    {
+        float m = 0.f;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const float2 x = lds[wave * RS + padj3(lane + P * e)];
+            m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (lane == 0) {
+            int ex = 0;
+            (void)frexpf(m, &ex);          // m = f 2^ex, 0.5 <= f < 1 (ex = 0 for m = 0)
+            lscale[wave] = ldexpf(1.0f, -ex);
+        }
+    }
+    __syncthreads();
+
+    const int j = lane & 31, h = lane >> 5;
+    // ---- step A: wave -> (column block b, real/imag output m); column j = (n2, line)
+    f32x16_t acc = {};
+    {
+        const int b = wave >> 1, m = wave & 1;
+        const int n2 = j & 15, line = 2 * b + (j >> 4);
+        const float s = lscale[line];
+        const int i = j;  // A row of this lane (k1)
+        float re[2][8], im[2][8];
+#pragma unroll
+        for (int half = 0; half < 2; half++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int n1 = 16 * half + 8 * h + e;
+                const float2 x = lds[line * RS + padj3(16 * n1 + n2)];
+                re[half][e] = x.x * s;
+                im[half][e] = x.y * s;
+            }
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++) {
+            float av[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int n1 = 16 * (ks & 1) + 8 * h + e;
+                const float2 w = w32[(i * n1) & 31];  // W32^{k1 n1}
+                av[e] = (m == 0) ? (ks < 2 ? w.x : -w.y) : (ks < 2 ? w.y : w.x);
+            }
+            vec A[3], B[3];
+            split8<PREC>(av, A);
+            split8<PREC>(ks < 2 ? re[ks & 1] : im[ks & 1], B);
+            acc = split_mma<PREC>(A, B, acc);
+        }
+    }
+    __syncthreads();  // every wave has read its operands: the tile may be overwritten
+    {
+        const int b = wave >> 1, m = wave & 1;
+        const int n2 = j & 15, line = 2 * b + (j >> 4);
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int k1 = (e & 3) + 8 * (e >> 2) + 4 * h;
+            ldsf[2 * (line * RS + padj3(k1 * 16 + n2)) + m] = acc[e];
+        }
+    }
+    __syncthreads();
+    // ---- twiddle + step C: wave -> 32 columns (k1, line)
+    int line, k1;
+    if (a.store_colfast) { line = j & 15; k1 = 2 * wave + (j >> 4); }
+    else { line = wave; k1 = j; }
+    acc = f32x16_t{};
+    {
+        float zr[8], zi[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int n2 = 8 * h + e;
+            const float2 y = lds[line * RS + padj3(k1 * 16 + n2)];
+            const float2 w = w512[n2 * k1];  // n2 * k1 <= 465
+            zr[e] = y.x * w.x - y.y * w.y;
+            zi[e] = y.x * w.y + y.y * w.x;
+        }
+        const int i = j;  // A row: i < 16 real part of X[k1 + 32 i], i >= 16 imaginary part of k2 = i - 16
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            float av[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int n2 = 8 * h + e;
+                const float2 w = w16[((i & 15) * n2) & 15];
+                av[e] = (i < 16) ? (ks == 0 ? w.x : -w.y) : (ks == 0 ? w.y : w.x);
+            }
+            vec A[3], B[3];
+            split8<PREC>(av, A);
+            split8<PREC>(ks == 0 ? zr : zi, B);
+            acc = split_mma<PREC>(A, B, acc);
+        }
+    }
+    // ---- store: lane holds X[k1 + 32 k2] for k2 = (e & 3) + 8 (e >> 2) + 4 h, e < 8
+    {
+        const unsigned L = tile * T + line;
+        if (L < (unsigned)a.nlines) {
+            const float s = (float)a.scale / lscale[line];  // undo the line scale
+            const long long base = line_off(L, a, true);
+            float2 *__restrict__ dst = static_cast<float2 *>(a.out);
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k2 = (e & 3) + 8 * (e >> 2) + 4 * h;
+                const int pos = k1 + 32 * k2;
+                dst[base + (long long)pos * a.out_sa] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
+            }
+        }
+    }
+}
+
+hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s) {
+    if (prec == kSplitF16) fft512_mfma<kSplitF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
+    else if (prec == kSplitBF16) fft512_mfma<kSplitBF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace hmi

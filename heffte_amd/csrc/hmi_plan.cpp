@@ -104,6 +104,7 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
 fft_plan::~fft_plan() {
     for (auto &kv : tw_)
         if (kv.second.dev) (void)hipFree(kv.second.dev);
+    if (mfma_tw_.dev) (void)hipFree(mfma_tw_.dev);
 }
 
 double fft_plan::scale_factor(int scaling) const {
@@ -521,7 +522,26 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.mode = p.mode;
         a.scale = p.last ? scale : 1.0;
         const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
-        if (p.fast) {
+        const bool mfma = precision_ != kPrecPlain && !dbl && p.n == 512 && p.mode == kModeC2C && !p.identity &&
+                          !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
+        if (mfma) {
+            // the split-precision form: W32, W16 and W512 tables, 16 lines per block
+            if (!mfma_tw_.dev) {
+                std::vector<float> t;
+                for (int m : {32, 16, 512})
+                    for (int r = 0; r < m; r++) {
+                        float re, im;
+                        root<float>(r, m, re, im);
+                        t.push_back(re);
+                        t.push_back(im);
+                    }
+                mfma_tw_.bytes = t.size() * sizeof(float);
+                check_hip(hipMalloc(&mfma_tw_.dev, mfma_tw_.bytes), "hipMalloc(twiddles)");
+                check_hip(hipMemcpy(mfma_tw_.dev, t.data(), mfma_tw_.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+            }
+            a.tw = mfma_tw_.dev;
+            check_hip(launch_mfma512(precision_, a, ((long long)a.nlines + 15) / 16, stream_), "MFMA FFT pass");
+        } else if (p.fast) {
             check_hip(p.fast->launch(a, nblocks, stream_), "fast FFT pass");
         } else {
             const size_t lds = 2 * (size_t)p.gen.lines * p.n * (dbl ? 16 : 8);

@@ -89,6 +89,9 @@ class fft_plan {
     double scale_factor(int scaling) const;
     void set_stream(hipStream_t s) { stream_ = s; }
     hipStream_t stream() const { return stream_; }
+    // kPrecPlain, kSplitF16 or kSplitBF16: the form of the 512-point complex<float> c2c passes
+    void set_precision(int p) { precision_ = p; }
+    int precision() const { return precision_; }
 
     // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
     // overloads of fft3d (real input on forward, real output on backward).
@@ -108,7 +111,9 @@ class fft_plan {
     int r2c_dir_;
     long long ws_elems_ = 0;
     hipStream_t stream_ = nullptr;
+    int precision_ = kPrecPlain;
     std::mutex mtx_;
+    twiddles mfma_tw_{};
     std::map<int, exec_plan> plans_;
     std::map<long long, twiddles> tw_;
 };

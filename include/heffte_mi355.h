@@ -97,6 +97,18 @@ int heffte_mi355_get_kind(heffte_mi355_plan plan);
  * the fft3d constructor (include/heffte_fft3d.h:287-290). */
 int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream);
 
+/* Precision form of the complex<float> c2c passes (no heFFTe counterpart):
+ *   HMI_PRECISION_FP32       vector-ALU FP32 (default)
+ *   HMI_PRECISION_SPLIT_FP16 512-point passes as 32x16 DFT GEMMs on MFMA, operands split
+ *                            into two FP16 terms (3 products), FP32 accumulation,
+ *                            per-line power-of-two scaling
+ *   HMI_PRECISION_SPLIT_BF16 the same with three BF16 terms (6 products)
+ * Other lengths and complex<double> always use the vector-ALU passes. */
+#define HMI_PRECISION_FP32 0
+#define HMI_PRECISION_SPLIT_FP16 1
+#define HMI_PRECISION_SPLIT_BF16 2
+int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision);
+
 /* Forward transforms; replace heffte_forward_{s2c,c2c,d2z,z2z}_buffered
  * (include/heffte_c.h:164-179) with a batch size.  workspace may be NULL, in
  * which case it is allocated for the call (fft3d::forward(in, out),

@@ -40,6 +40,7 @@ def parse():
     p.add_argument("--no-rocfft", action="store_true", help="skip the rocFFT comparison")
     p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     p.add_argument("--no-batched", action="store_true", help="skip the 1000-transform mixed-size batch leg")
+    p.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA forms")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return p.parse_args()
@@ -210,6 +211,31 @@ def main():
                      "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": bytes_per_launch},
     }
 
+    # split-precision MFMA forms of the same transform (HMI_PRECISION_SPLIT_FP16 / _BF16)
+    if args.dtype == "complex64" and n == 512 and not args.no_split:
+        forms = {}
+        for name, form in (("split_fp16", heffte.precision.split_fp16), ("split_bf16", heffte.precision.split_bf16)):
+            plan.set_precision(form)
+            for _ in range(args.warmup):
+                step()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(args.steps):
+                step()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / args.steps
+            entry = {"ms_per_pair": round(ms, 4), "speedup_vs_fp32": round(ms_pair / ms, 3),
+                     "hbm_frac": round(bytes_per_launch / (ms / launches_per_pair * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            if x0 is not None:
+                yf = torch.empty_like(x0)
+                plan.forward_buffered(x0, yf, ws, heffte.scale.none)
+                entry["_y"] = yf
+            forms[name] = entry
+        plan.set_precision(heffte.precision.fp32)
+        out["split_precision"] = forms
+
     if args.calibrate_copy:
         # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
         # calibrate the counters read for the FFT passes in the same profiled run
@@ -234,6 +260,11 @@ def main():
         num = torch.linalg.vector_norm(y.to(torch.complex128) - y64).item()
         den = torch.linalg.vector_norm(y64).item()
         out["rel_l2_vs_fp64"] = num / den
+        for entry in out.get("split_precision", {}).values():
+            yf = entry.pop("_y", None)
+            if yf is not None:
+                entry["rel_l2_vs_fp64"] = torch.linalg.vector_norm(yf.to(torch.complex128) - y64).item() / den
+                del yf
         del y, x64, y64, ws64
         torch.cuda.empty_cache()
 

@@ -16,6 +16,10 @@
 // element (i % 4) + 4 (i / 8).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "hmi_device.h"
 #include "hmi_kernels.h"
 
@@ -96,23 +100,27 @@ __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bo
 
 }  // namespace
 
-// One block = 16 lines x 512 points, 1024 threads (16 waves).  The twiddle table holds
-// W32^r (32 entries), W16^r (16) and W512^r (512), complex<float>, rounded from long double.
+// One block = 16 lines x 512 points, 1024 threads (16 waves).  a.tw holds the W512^r table
+// (512 complex<float>) followed by the A operands of both GEMMs, already split and laid out
+// per lane (mfma512_operand_table): step A [m][ks][term][lane][8], step C [ks][term][lane][8].
 template <int PREC>
-__global__ void __launch_bounds__(1024) fft512_mfma(pass_args a) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
+fft512_mfma(pass_args a) {
     constexpr int N = 512, T = 16, P = 64, E = 8;
     constexpr int RS = N + (N >> 3) + 1;
+    using S = split_traits<PREC>;
+    using vec = typename S::vec;
+    constexpr int TERMS = S::terms;
     __shared__ float2 lds[T * RS];
     __shared__ float lscale[T];
-    using vec = typename split_traits<PREC>::vec;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const unsigned tile = blockIdx.x;
     const float sg = a.conj ? -1.0f : 1.0f;
-    const float2 *__restrict__ w32 = static_cast<const float2 *>(a.tw);
-    const float2 *__restrict__ w16 = w32 + 32;
-    const float2 *__restrict__ w512 = w32 + 48;
+    const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
+    const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);  // [2][4][TERMS][64]
+    const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;                    // [2][TERMS][64]
     float *ldsf = reinterpret_cast<float *>(lds);
 
     // ---- load the tile (coalesced along the axis or along the lines), conjugate for backward
@@ -144,43 +152,38 @@ __global__ void __launch_bounds__(1024) fft512_mfma(pass_args a) {
         for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
         if (lane == 0) {
             int ex = 0;
-            (void)frexpf(m, &ex);          // m = f 2^ex, 0.5 <= f < 1 (ex = 0 for m = 0)
+            (void)frexpf(m, &ex);  // m = f 2^ex with 0.5 <= f < 1 (ex = 0 for m = 0)
             lscale[wave] = ldexpf(1.0f, -ex);
         }
     }
     __syncthreads();
 
     const int j = lane & 31, h = lane >> 5;
-    // ---- step A: wave -> (column block b, real/imag output m); column j = (n2, line)
+    // ---- step A: wave -> (column block b, real/imaginary output m); column j = (n2, line)
     f32x16_t acc = {};
     {
         const int b = wave >> 1, m = wave & 1;
         const int n2 = j & 15, line = 2 * b + (j >> 4);
         const float s = lscale[line];
-        const int i = j;  // A row of this lane (k1)
-        float re[2][8], im[2][8];
 #pragma unroll
-        for (int half = 0; half < 2; half++)
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int n1 = 16 * half + 8 * h + e;
-                const float2 x = lds[line * RS + padj3(16 * n1 + n2)];
-                re[half][e] = x.x * s;
-                im[half][e] = x.y * s;
-            }
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            float av[8];
+        for (int half = 0; half < 2; half++) {
+            float re[8], im[8];
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const int n1 = 16 * (ks & 1) + 8 * h + e;
-                const float2 w = w32[(i * n1) & 31];  // W32^{k1 n1}
-                av[e] = (m == 0) ? (ks < 2 ? w.x : -w.y) : (ks < 2 ? w.y : w.x);
+                const float2 x = lds[line * RS + padj3(16 * (16 * half + 8 * h + e) + n2)];
+                re[e] = x.x * s;
+                im[e] = x.y * s;
             }
-            vec A[3], B[3];
-            split8<PREC>(av, A);
-            split8<PREC>(ks < 2 ? re[ks & 1] : im[ks & 1], B);
-            acc = split_mma<PREC>(A, B, acc);
+            // K-step `half` multiplies the real parts, K-step `2 + half` the imaginary parts
+#pragma unroll
+            for (int part = 0; part < 2; part++) {
+                const int ks = half + 2 * part;
+                vec A[3], B[3];
+#pragma unroll
+                for (int tt = 0; tt < TERMS; tt++) A[tt] = opA[((m * 4 + ks) * TERMS + tt) * 64 + lane];
+                split8<PREC>(part == 0 ? re : im, B);
+                acc = split_mma<PREC>(A, B, acc);
+            }
         }
     }
     __syncthreads();  // every wave has read its operands: the tile may be overwritten
@@ -209,18 +212,11 @@ __global__ void __launch_bounds__(1024) fft512_mfma(pass_args a) {
             zr[e] = y.x * w.x - y.y * w.y;
             zi[e] = y.x * w.y + y.y * w.x;
         }
-        const int i = j;  // A row: i < 16 real part of X[k1 + 32 i], i >= 16 imaginary part of k2 = i - 16
 #pragma unroll
         for (int ks = 0; ks < 2; ks++) {
-            float av[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int n2 = 8 * h + e;
-                const float2 w = w16[((i & 15) * n2) & 15];
-                av[e] = (i < 16) ? (ks == 0 ? w.x : -w.y) : (ks == 0 ? w.y : w.x);
-            }
             vec A[3], B[3];
-            split8<PREC>(av, A);
+#pragma unroll
+            for (int tt = 0; tt < TERMS; tt++) A[tt] = opC[(ks * TERMS + tt) * 64 + lane];
             split8<PREC>(ks == 0 ? zr : zi, B);
             acc = split_mma<PREC>(A, B, acc);
         }
@@ -240,6 +236,69 @@ __global__ void __launch_bounds__(1024) fft512_mfma(pass_args a) {
             }
         }
     }
+}
+
+namespace {
+long double cpi() { return 3.141592653589793238462643383279502884L; }
+// exp(-2 pi i r / m) in long double
+void wroot(long long r, long long m, long double &re, long double &im) {
+    const long double ang = 2.0L * cpi() * (long double)(r % m) / (long double)m;
+    re = cosl(ang);
+    im = -sinl(ang);
+}
+template <typename S> void split_store(float v, int terms, S *out, int stride) {
+    float r = v;
+    for (int t = 0; t < terms; t++) {
+        const S p = (S)r;
+        out[t * stride] = p;
+        r -= (float)p;
+    }
+}
+template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int terms) {
+    // step A: A[i][k] of [Re F32, -Im F32; Im F32, Re F32], rows i = k1 (m = 0: real rows,
+    // m = 1: imaginary rows), K-step ks covers k = 16 ks + 8 (lane / 32) + e
+    const size_t nA = 2 * 4 * terms * 64 * 8, nC = 2 * terms * 64 * 8;
+    std::vector<S> A(nA), Cm(nC);
+    for (int m = 0; m < 2; m++)
+        for (int ks = 0; ks < 4; ks++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int e = 0; e < 8; e++) {
+                    const int i = lane & 31, n1 = 16 * (ks & 1) + 8 * (lane >> 5) + e;
+                    long double wr, wi;
+                    wroot((long long)i * n1, 32, wr, wi);
+                    const long double v = (m == 0) ? (ks < 2 ? wr : -wi) : (ks < 2 ? wi : wr);
+                    split_store<S>((float)v, terms, &A[(((size_t)(m * 4 + ks) * terms) * 64 + lane) * 8 + e], 64 * 8);
+                }
+    // step C: rows i < 16 real part of X[k1 + 32 i], i >= 16 imaginary part of k2 = i - 16;
+    // K-step 0 multiplies Re z, K-step 1 Im z, n2 = 8 (lane / 32) + e
+    for (int ks = 0; ks < 2; ks++)
+        for (int lane = 0; lane < 64; lane++)
+            for (int e = 0; e < 8; e++) {
+                const int i = lane & 31, n2 = 8 * (lane >> 5) + e;
+                long double wr, wi;
+                wroot((long long)(i & 15) * n2, 16, wr, wi);
+                const long double v = (i < 16) ? (ks == 0 ? wr : -wi) : (ks == 0 ? wi : wr);
+                split_store<S>((float)v, terms, &Cm[(((size_t)ks * terms) * 64 + lane) * 8 + e], 64 * 8);
+            }
+    const size_t off = bytes.size();
+    bytes.resize(off + (nA + nC) * sizeof(S));
+    std::memcpy(bytes.data() + off, A.data(), nA * sizeof(S));
+    std::memcpy(bytes.data() + off + nA * sizeof(S), Cm.data(), nC * sizeof(S));
+}
+}  // namespace
+
+std::vector<unsigned char> mfma512_operand_table(int prec) {
+    std::vector<unsigned char> bytes(512 * 2 * sizeof(float));
+    float *w = reinterpret_cast<float *>(bytes.data());
+    for (int r = 0; r < 512; r++) {
+        long double re, im;
+        wroot(r, 512, re, im);
+        w[2 * r] = (float)re;
+        w[2 * r + 1] = (float)im;
+    }
+    if (prec == kSplitF16) fill_operands<_Float16>(bytes, 2);
+    else fill_operands<__bf16>(bytes, 3);
+    return bytes;
 }
 
 hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s) {

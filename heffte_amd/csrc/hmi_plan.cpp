@@ -104,7 +104,8 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
 fft_plan::~fft_plan() {
     for (auto &kv : tw_)
         if (kv.second.dev) (void)hipFree(kv.second.dev);
-    if (mfma_tw_.dev) (void)hipFree(mfma_tw_.dev);
+    for (auto &t : mfma_tw_)
+        if (t.dev) (void)hipFree(t.dev);
 }
 
 double fft_plan::scale_factor(int scaling) const {
@@ -526,20 +527,14 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
                           !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
         if (mfma) {
             // the split-precision form: W32, W16 and W512 tables, 16 lines per block
-            if (!mfma_tw_.dev) {
-                std::vector<float> t;
-                for (int m : {32, 16, 512})
-                    for (int r = 0; r < m; r++) {
-                        float re, im;
-                        root<float>(r, m, re, im);
-                        t.push_back(re);
-                        t.push_back(im);
-                    }
-                mfma_tw_.bytes = t.size() * sizeof(float);
-                check_hip(hipMalloc(&mfma_tw_.dev, mfma_tw_.bytes), "hipMalloc(twiddles)");
-                check_hip(hipMemcpy(mfma_tw_.dev, t.data(), mfma_tw_.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+            const int pi = precision_ == kSplitF16 ? 0 : 1;
+            if (!mfma_tw_[pi].dev) {
+                std::vector<unsigned char> t = mfma512_operand_table(precision_);
+                mfma_tw_[pi].bytes = t.size();
+                check_hip(hipMalloc(&mfma_tw_[pi].dev, t.size()), "hipMalloc(twiddles)");
+                check_hip(hipMemcpy(mfma_tw_[pi].dev, t.data(), t.size(), hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
             }
-            a.tw = mfma_tw_.dev;
+            a.tw = mfma_tw_[pi].dev;
             check_hip(launch_mfma512(precision_, a, ((long long)a.nlines + 15) / 16, stream_), "MFMA FFT pass");
         } else if (p.fast) {
             check_hip(p.fast->launch(a, nblocks, stream_), "fast FFT pass");

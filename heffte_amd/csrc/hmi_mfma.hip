@@ -87,8 +87,6 @@ __device__ __forceinline__ f32x16_t split_mma(const typename split_traits<PREC>:
     return acc;
 }
 
-__device__ __forceinline__ int padj3(int j) { return j + (j >> 3); }
-
 __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bool out) {
     const unsigned q = L / (unsigned)a.ncol;
     const unsigned c = L - q * (unsigned)a.ncol;
@@ -103,16 +101,22 @@ __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bo
 // One block = 16 lines x 512 points, 1024 threads (16 waves).  a.tw holds the W512^r table
 // (512 complex<float>) followed by the A operands of both GEMMs, already split and laid out
 // per lane (mfma512_operand_table): step A [m][ks][term][lane][8], step C [ks][term][lane][8].
+//
+// LDS holds the tile twice over its life, in the layout each GEMM reads with contiguous
+// K: x[16 n1 + n2] at LA(line, n2, n1) for step A, Y[k1][n2] at LC(line, k1, n2) for step C
+// (odd row pitches spread the rows over the banks).
 template <int PREC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
 fft512_mfma(pass_args a) {
-    constexpr int N = 512, T = 16, P = 64, E = 8;
-    constexpr int RS = N + (N >> 3) + 1;
+    constexpr int T = 16, P = 64, E = 8;
+    constexpr int RA = 33, RLA = 16 * RA + 1;  // step A rows: n2 x 33
+    constexpr int RC = 17, RLC = 32 * RC + 1;  // step C rows: k1 x 17
+    constexpr int LDS = (T * RLA > T * RLC) ? T * RLA : T * RLC;
     using S = split_traits<PREC>;
     using vec = typename S::vec;
     constexpr int TERMS = S::terms;
-    __shared__ float2 lds[T * RS];
-    __shared__ float lscale[T];
+    __shared__ float2 lds[LDS];
+    __shared__ unsigned lmax[T];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -123,38 +127,38 @@ fft512_mfma(pass_args a) {
     const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;                    // [2][TERMS][64]
     float *ldsf = reinterpret_cast<float *>(lds);
 
-    // ---- load the tile (coalesced along the axis or along the lines), conjugate for backward
+    if (tid < T) lmax[tid] = 0u;
+    __syncthreads();
+    // ---- load the tile into the step-A layout; per-line maximum of |re|, |im|
     {
         int l, t;
         if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
         const unsigned L = tile * T + l;
         const bool valid = L < (unsigned)a.nlines;
         const long long base = valid ? line_off(L, a, false) : 0;
-        const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
+        // all eight loads in flight before the first use (an invalid line reads line 0)
+        const float2 *__restrict__ p = static_cast<const float2 *>(a.in) + base + (long long)t * a.in_sa;
+        const long long step = (long long)P * a.in_sa;
+        float2 x[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int pos = t + P * e;
-            float2 x = valid ? src[base + (long long)pos * a.in_sa] : make_float2(0.f, 0.f);
-            x.y *= sg;
-            lds[l * RS + padj3(pos)] = x;
-        }
-    }
-    __syncthreads();
-    // ---- per-line power-of-two scale: wave w reduces line w
-    {
+        for (int e = 0; e < E; e++) x[e] = p[e * step];
         float m = 0.f;
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            const float2 x = lds[wave * RS + padj3(lane + P * e)];
-            m = fmaxf(m, fmaxf(fabsf(x.x), fabsf(x.y)));
+            const int pos = t + P * e;
+            if (!valid) x[e] = make_float2(0.f, 0.f);
+            x[e].y *= sg;
+            m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
+            lds[l * RLA + (pos & 15) * RA + (pos >> 4)] = x[e];
         }
+        // lanes holding the same line: all 64 (along the axis) or lanes l, l+16, l+32, l+48
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        if (!a.load_colfast) {
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if (lane == 0) {
-            int ex = 0;
-            (void)frexpf(m, &ex);  // m = f 2^ex with 0.5 <= f < 1 (ex = 0 for m = 0)
-            lscale[wave] = ldexpf(1.0f, -ex);
+            for (int off = 8; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
         }
+        if ((a.load_colfast && lane < 16) || (!a.load_colfast && lane == 0)) atomicMax(&lmax[l], __float_as_uint(m));
     }
     __syncthreads();
 
@@ -164,15 +168,21 @@ fft512_mfma(pass_args a) {
     {
         const int b = wave >> 1, m = wave & 1;
         const int n2 = j & 15, line = 2 * b + (j >> 4);
-        const float s = lscale[line];
+        int ex = 0;
+        (void)frexpf(__uint_as_float(lmax[line]), &ex);  // line max = f 2^ex, 0.5 <= f < 1
+        const float s = ldexpf(1.0f, -ex);
 #pragma unroll
         for (int half = 0; half < 2; half++) {
             float re[8], im[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const float2 x = lds[line * RS + padj3(16 * (16 * half + 8 * h + e) + n2)];
-                re[e] = x.x * s;
-                im[e] = x.y * s;
+            for (int q = 0; q < 4; q++) {
+                // n1 = 16 half + 8 h + 2 q, +1: two complex values per 16-byte read
+                const float2 *p2 = &lds[line * RLA + n2 * RA + 16 * half + 8 * h + 2 * q];
+                const float2 x0 = p2[0], x1 = p2[1];
+                re[2 * q] = x0.x * s;
+                im[2 * q] = x0.y * s;
+                re[2 * q + 1] = x1.x * s;
+                im[2 * q + 1] = x1.y * s;
             }
             // K-step `half` multiplies the real parts, K-step `2 + half` the imaginary parts
 #pragma unroll
@@ -193,7 +203,7 @@ fft512_mfma(pass_args a) {
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int k1 = (e & 3) + 8 * (e >> 2) + 4 * h;
-            ldsf[2 * (line * RS + padj3(k1 * 16 + n2)) + m] = acc[e];
+            ldsf[2 * (line * RLC + k1 * RC + n2) + m] = acc[e];
         }
     }
     __syncthreads();
@@ -207,7 +217,7 @@ fft512_mfma(pass_args a) {
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int n2 = 8 * h + e;
-            const float2 y = lds[line * RS + padj3(k1 * 16 + n2)];
+            const float2 y = lds[line * RLC + k1 * RC + n2];
             const float2 w = w512[n2 * k1];  // n2 * k1 <= 465
             zr[e] = y.x * w.x - y.y * w.y;
             zi[e] = y.x * w.y + y.y * w.x;
@@ -225,14 +235,16 @@ fft512_mfma(pass_args a) {
     {
         const unsigned L = tile * T + line;
         if (L < (unsigned)a.nlines) {
-            const float s = (float)a.scale / lscale[line];  // undo the line scale
-            const long long base = line_off(L, a, true);
-            float2 *__restrict__ dst = static_cast<float2 *>(a.out);
+            int ex = 0;
+            (void)frexpf(__uint_as_float(lmax[line]), &ex);
+            const float s = ldexpf((float)a.scale, ex);  // undo the line scale
+            float2 *__restrict__ p = static_cast<float2 *>(a.out) + line_off(L, a, true) +
+                                     (long long)(k1 + 128 * h) * a.out_sa;
+            const long long step = 32 * a.out_sa;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
-                const int k2 = (e & 3) + 8 * (e >> 2) + 4 * h;
-                const int pos = k1 + 32 * k2;
-                dst[base + (long long)pos * a.out_sa] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
+                const int k2 = (e & 3) + 8 * (e >> 2);  // + 4 h, folded into p
+                p[k2 * step] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
             }
         }
     }

@@ -120,6 +120,7 @@ class kind:
     r2c = 1
     cos = 2
     sin = 3
+    cos1 = 4  # DCT-I, the convention of backend::rocfft_cos1
 
 
 class box3d:
@@ -216,7 +217,7 @@ class fft_plan:
         if scaling not in (0, 1, 2):
             raise heffte_input_error(f"{direction}() called with invalid scaling")
         ti, to = _dtype_name(inarray), _dtype_name(outarray)
-        if self.kind in (kind.cos, kind.sin):
+        if self.kind in (kind.cos, kind.sin, kind.cos1):
             table = {("float32", "float32"): "s2s", ("float64", "float64"): "d2d"}
         elif direction == "forward":
             table = {("float32", "complex64"): "s2c", ("float64", "complex128"): "d2z"}
@@ -290,7 +291,7 @@ def fft3d_r2c(backend_tag, inbox, outbox, r2c_direction, comm=None):
 
 
 def rtransform(backend_tag, inbox, outbox, transform=kind.cos, comm=None):
-    """heffte::rtransform<backend::*_cos / *_sin>."""
+    """heffte::rtransform<backend::*_cos / *_sin / *_cos1> (transform = kind.cos, kind.sin, kind.cos1)."""
     _check_backend(backend_tag)
     _check_comm(comm)
     h = ctypes.c_void_p()

@@ -83,7 +83,7 @@ int run_checked(heffte_mi355_plan plan, bool r2r, int dir, bool dbl, bool in_rea
                 const void *in, void *out, void *ws, int scale) {
     if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
     const hmi::kind k = plan->impl->get_kind();
-    const bool is_r2r = (k == hmi::kind::cos || k == hmi::kind::sin);
+    const bool is_r2r = (k == hmi::kind::cos || k == hmi::kind::sin || k == hmi::kind::cos1);
     if (r2r != is_r2r)
         return fail(HMI_ERROR_INVALID_ARGUMENT, r2r ? "real-to-real call on a complex plan"
                                                     : "complex call on a real-to-real plan");
@@ -126,12 +126,12 @@ int heffte_mi355_plan_create_r2r(int kind, long long const inbox_low[3], long lo
                                  int const *inbox_order, long long const outbox_low[3],
                                  long long const outbox_high[3], int const *outbox_order,
                                  heffte_mi355_plan_options const *, heffte_mi355_plan *plan) {
-    if (kind != HMI_KIND_COS && kind != HMI_KIND_SIN) {
+    if (kind != HMI_KIND_COS && kind != HMI_KIND_SIN && kind != HMI_KIND_COS1) {
         if (plan) *plan = nullptr;
-        return fail(HMI_ERROR_INVALID_ARGUMENT, "r2r kind must be HMI_KIND_COS or HMI_KIND_SIN");
+        return fail(HMI_ERROR_INVALID_ARGUMENT, "r2r kind must be HMI_KIND_COS, HMI_KIND_SIN or HMI_KIND_COS1");
     }
-    return create(kind == HMI_KIND_COS ? hmi::kind::cos : hmi::kind::sin, inbox_low, inbox_high, inbox_order,
-                  outbox_low, outbox_high, outbox_order, -1, plan);
+    const hmi::kind k = kind == HMI_KIND_COS ? hmi::kind::cos : (kind == HMI_KIND_SIN ? hmi::kind::sin : hmi::kind::cos1);
+    return create(k, inbox_low, inbox_high, inbox_order, outbox_low, outbox_high, outbox_order, -1, plan);
 }
 
 int heffte_mi355_plan_destroy(heffte_mi355_plan plan) {

@@ -179,6 +179,15 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
                 const C W = cmk<C>(w.x * xk - w.y * xm, -w.x * xm - w.y * xk);
                 v[e] = cmk<C>(W.x, -W.y);
             }
+        } else if (a.mode == kModeDCT1 || a.mode == kModeDCT1B) {
+            // even extension of N = F/2 + 1 reals to the F-point line
+            const R *__restrict__ src = static_cast<const R *>(a.in);
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int j = t + P * e;
+                const int jj = (j <= N / 2) ? j : N - j;
+                v[e] = cmk<C>(valid ? src[base + (long long)jj * a.in_sa] : R(0), R(0));
+            }
         } else if (a.in_real) {
             const R *__restrict__ src = static_cast<const R *>(a.in);
 #pragma unroll
@@ -260,6 +269,12 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             if (sine && (j & 1)) y = -y;
             dst[obase + (long long)j * a.out_sa] = y * s;
         }
+    } else if (a.mode == kModeDCT1 || a.mode == kModeDCT1B) {
+        R *__restrict__ dst = static_cast<R *>(a.out);
+        const R f = (a.mode == kModeDCT1B) ? R(2) * s : s;
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            if (t2 + P * e < a.store_len) dst[obase + (long long)(t2 + P * e) * a.out_sa] = v[e].x * f;
     } else if (a.mode == kModeC2R) {
         R *__restrict__ dst = static_cast<R *>(a.out);
 #pragma unroll
@@ -341,6 +356,8 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
                 const R xm = j > 0 ? rsrc[lbase + (long long)(sine ? j - 1 : N - j) * a.in_sa] : R(0);
                 const C w = static_cast<const C *>(a.tw)[g.postoff + j];
                 x = cmk<C>(w.x * xk - w.y * xm, w.x * xm + w.y * xk);  // conj of conj(w) (xk - i xm)
+            } else if (a.mode == kModeDCT1 || a.mode == kModeDCT1B) {
+                x.x = rsrc[lbase + (long long)(j <= N / 2 ? j : N - j) * a.in_sa];
             } else if (a.in_real) {
                 x.x = static_cast<const R *>(a.in)[off];
             } else if (a.herm_load && j > N / 2) {
@@ -416,6 +433,8 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
             R r = R(2) * y.x;
             if (a.mode == kModeDST3 && (jj & 1)) r = -r;
             static_cast<R *>(a.out)[lobase + (long long)jj * a.out_sa] = r * sc;
+        } else if (a.mode == kModeDCT1B) {
+            static_cast<R *>(a.out)[off] = R(2) * y.x * sc;
         } else if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
         else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
     }

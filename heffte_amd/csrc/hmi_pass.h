@@ -27,7 +27,14 @@ enum : int { kMaxStages = 24 };
 //                             y[2m] = 2 Re w[m], y[2m+1] = 2 Re w[N-1-m]
 //   kModeDST2 / kModeDST3: the same with the sine identities
 //     DST-II(x)[k] = DCT-II((-1)^n x[n])[N-1-k],  DST-III(x)[n] = (-1)^n DCT-III(x[N-1-.])[n]
-enum : int { kModeC2C = 0, kModeR2C = 1, kModeC2R = 2, kModeDCT2 = 3, kModeDCT3 = 4, kModeDST2 = 5, kModeDST3 = 6 };
+//
+// DCT-I (rtransform<cos1>) runs the kernel's FFT length F = 2(N-1) on the even extension
+// E[j] = x[j] (j <= N-1), x[F-j] (j > N-1), imaginary parts zero, and stores y[k] = Re X[k]
+// for k < N (store_len = N): kModeDCT1 forward (REDFT00), kModeDCT1B backward (2 REDFT00).
+enum : int {
+    kModeC2C = 0, kModeR2C = 1, kModeC2R = 2, kModeDCT2 = 3, kModeDCT3 = 4, kModeDST2 = 5, kModeDST3 = 6,
+    kModeDCT1 = 7, kModeDCT1B = 8
+};
 
 struct pass_args {
     const void *in;

@@ -84,6 +84,9 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
     } else {
         if (!inbox.same_indexes(outbox))
             throw std::invalid_argument("inbox and outbox must hold the same indexes on a single rank");
+        if (k == kind::cos1)
+            for (int d = 0; d < 3; d++)
+                if (inbox.size(d) < 2) throw std::invalid_argument("DCT-I needs at least 2 points along every axis");
     }
     // workspace: the largest over every variant the plan may run
     long long ws = 0;
@@ -94,7 +97,7 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
                     if (k == kind::c2c && ir && dir == 1) continue;
                     if (k == kind::c2c && orl && dir == -1) continue;
                     if (k == kind::r2c && (ir != (dir == -1) || orl != (dir == 1))) continue;
-                    if ((k == kind::cos || k == kind::sin) && (!ir || !orl)) continue;
+                    if ((k == kind::cos || k == kind::sin || k == kind::cos1) && (!ir || !orl)) continue;
                     exec_plan p = build(dir, false, ir, orl, ip);
                     ws = std::max(ws, p.ws_elems);
                 }
@@ -113,8 +116,11 @@ double fft_plan::scale_factor(int scaling) const {
     const double n = (double)inbox_.count();
     double s = 1.0 / n;
     // rtransform of a GPU/stock backend: round trip 4 n per axis, 64 N in 3D
-    // (include/heffte_fft3d.h:635-650, the branch every non-FFTW r2r tag takes)
+    // (include/heffte_fft3d.h:635-650, the branch every non-FFTW r2r tag takes);
+    // DCT-I: round trip 4 (n - 1) per axis, 1 / (64 prod(n - 1)) (:643-646)
     if (kind_ == kind::cos || kind_ == kind::sin) s /= 64.0;
+    if (kind_ == kind::cos1)
+        s = 1.0 / (64.0 * (double)(inbox_.size(0) - 1) * (double)(inbox_.size(1) - 1) * (double)(inbox_.size(2) - 1));
     if (scaling == 0) return 1.0;
     if (scaling == 2) return std::sqrt(s);
     return s;
@@ -142,7 +148,7 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
     for (int d = 0; d < 3; d++) nfull[d] = inbox_.size(d);
 
     // axes to transform
-    const bool r2r = (kind_ == kind::cos || kind_ == kind::sin);
+    const bool r2r = (kind_ == kind::cos || kind_ == kind::sin || kind_ == kind::cos1);
     std::vector<int> axes;
     if (r2c && fwd) axes.push_back(r2c_dir_);
     for (int d = 0; d < 3; d++)
@@ -188,6 +194,11 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                         else p.herm_load = 1;
                     }
                     if (!p.store_len) p.store_len = p.n;
+                    if (kind_ == kind::cos1 && !cp.identity) {
+                        // the kernel runs the 2(N-1)-point even extension and stores N values
+                        p.store_len = p.n;
+                        p.n = 2 * (p.n - 1);
+                    }
                     p.identity = cp.identity;
                     p.lin = cur;
                     p.src = cur_buf;
@@ -202,8 +213,10 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                     }
                     p.in_real = (r2r || (i == 0 && in_real)) ? 1 : 0;
                     p.out_real = (r2r || (p.last && out_real)) ? 1 : 0;
-                    if (r2r && !cp.identity)
-                        p.mode = (kind_ == kind::cos) ? (fwd ? kModeDCT2 : kModeDCT3) : (fwd ? kModeDST2 : kModeDST3);
+                    if (r2r && !cp.identity) {
+                        if (kind_ == kind::cos1) p.mode = fwd ? kModeDCT1 : kModeDCT1B;
+                        else p.mode = (kind_ == kind::cos) ? (fwd ? kModeDCT2 : kModeDCT3) : (fwd ? kModeDST2 : kModeDST3);
+                    }
                     // an intermediate in OUT must fit the output buffer as complex data
                     if (!p.last && p.dst == buf::out) {
                         const long long cnt = odims[0] * odims[1] * odims[2];
@@ -236,7 +249,8 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                                                         (p.src == buf::out && p.dst == buf::in)));
                     if (same_mem) {
                         // r2r passes read and write reals: same element type on both sides
-                        bool preserve = (r2r || (!p.in_real && !p.out_real)) && !p.herm_load && p.store_len == p.n;
+                        bool preserve = (r2r || (!p.in_real && !p.out_real)) && !p.herm_load &&
+                                        (p.store_len == p.n || p.mode == kModeDCT1 || p.mode == kModeDCT1B);
                         for (int d = 0; d < 3 && preserve; d++)
                             if (dims[d] != odims[d] || (dims[d] > 1 && p.lin.stride[d] != p.lout.stride[d]))
                                 preserve = false;
@@ -440,7 +454,7 @@ const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     int tail_count = 0;
     long long tail_mod = 1;
     if (p.mode == kModeR2C || p.mode == kModeC2R) { tail_count = n / 2; tail_mod = n; }
-    if (p.mode >= kModeDCT2) { tail_count = n; tail_mod = 4LL * n; }
+    if (p.mode >= kModeDCT2 && p.mode <= kModeDST3) { tail_count = n; tail_mod = 4LL * n; }
     if (dbl) {
         std::vector<double> v = make_table<double>(radix, ns, !p.fast, tail_count, tail_mod);
         t.bytes = v.size() * sizeof(double);

@@ -20,7 +20,7 @@
 
 namespace hmi {
 
-enum class kind { c2c = 0, r2c = 1, cos = 2, sin = 3 };
+enum class kind { c2c = 0, r2c = 1, cos = 2, sin = 3, cos1 = 4 };
 enum class buf { in = 0, out = 1, ws = 2 };
 
 struct box3 {

@@ -45,6 +45,7 @@ extern "C" {
 #define HMI_KIND_R2C 1 /* heffte::fft3d_r2c                               */
 #define HMI_KIND_COS 2 /* heffte::rtransform<backend::*_cos> (DCT-II/III) */
 #define HMI_KIND_SIN 3 /* heffte::rtransform<backend::*_sin> (DST-II/III) */
+#define HMI_KIND_COS1 4 /* heffte::rtransform<backend::rocfft_cos1> (DCT-I) */
 
 /* Same layout as heffte_plan_options (include/heffte_c_defines.h:113-122).
  * On a single rank the options do not change any result; they are accepted
@@ -74,7 +75,9 @@ int heffte_mi355_plan_create_r2c(long long const inbox_low[3], long long const i
                                  heffte_mi355_plan_options const *options, heffte_mi355_plan *plan);
 
 /* Replaces the construction of heffte::rtransform<tag> (include/heffte_fft3d.h:748-749;
- * the reference C binding has no r2r entry).  kind is HMI_KIND_COS or HMI_KIND_SIN. */
+ * the reference C binding has no r2r entry).  kind is HMI_KIND_COS, HMI_KIND_SIN or
+ * HMI_KIND_COS1; DCT-I needs at least 2 points along every axis and scales by
+ * 1 / (64 prod(n_d - 1)) under HMI_SCALE_FULL (include/heffte_fft3d.h:643-646). */
 int heffte_mi355_plan_create_r2r(int kind, long long const inbox_low[3], long long const inbox_high[3],
                                  int const *inbox_order, long long const outbox_low[3],
                                  long long const outbox_high[3], int const *outbox_order,
@@ -133,7 +136,7 @@ int heffte_mi355_backward_c2s(heffte_mi355_plan plan, int batch, void const *inp
 int heffte_mi355_backward_z2d(heffte_mi355_plan plan, int batch, void const *input, double *output, void *workspace,
                               int scale);
 
-/* Real-to-real transforms of an HMI_KIND_COS/SIN plan; replace
+/* Real-to-real transforms of an HMI_KIND_COS/SIN/COS1 plan; replace
  * rtransform::forward/backward(float/double data, workspace)
  * (include/heffte_fft3d.h:374-384,480-490 with a real scalar type). */
 int heffte_mi355_forward_s2s(heffte_mi355_plan plan, int batch, float const *input, float *output, void *workspace,

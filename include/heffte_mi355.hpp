@@ -100,16 +100,20 @@ struct mi355 {};
 struct mi355_cos {};
 //! rtransform: DST-II forward / DST-III backward (the convention of backend::rocfft_sin).
 struct mi355_sin {};
+//! rtransform: DCT-I forward and backward (the convention of backend::rocfft_cos1).
+struct mi355_cos1 {};
 
 template <typename tag> struct is_enabled : std::false_type {};
 template <> struct is_enabled<mi355> : std::true_type {};
 template <> struct is_enabled<mi355_cos> : std::true_type {};
 template <> struct is_enabled<mi355_sin> : std::true_type {};
+template <> struct is_enabled<mi355_cos1> : std::true_type {};
 
 template <typename tag> inline std::string name() { return "unknown"; }
 template <> inline std::string name<mi355>() { return "mi355"; }
 template <> inline std::string name<mi355_cos>() { return "mi355-cos"; }
 template <> inline std::string name<mi355_sin>() { return "mi355-sin"; }
+template <> inline std::string name<mi355_cos1>() { return "mi355-cos1"; }
 }  // namespace backend
 
 //! default_options<tag>: the fused executor absorbs the layout permutation, no reorder pass.
@@ -234,13 +238,14 @@ template <typename T> struct real_of<std::complex<T>> { using type = T; };
 template <typename tag> struct plan_kind { static constexpr int value = HMI_KIND_C2C; };
 template <> struct plan_kind<backend::mi355_cos> { static constexpr int value = HMI_KIND_COS; };
 template <> struct plan_kind<backend::mi355_sin> { static constexpr int value = HMI_KIND_SIN; };
+template <> struct plan_kind<backend::mi355_cos1> { static constexpr int value = HMI_KIND_COS1; };
 
 inline int to_c(scale s) { return s == scale::none ? HMI_SCALE_NONE : (s == scale::full ? HMI_SCALE_FULL : HMI_SCALE_SYMMETRIC); }
 }  // namespace detail
 
 /*!
  * \brief Single-rank 3D FFT (heffte::fft3d, include/heffte_fft3d.h:239-692), with
- *        backend::mi355 (complex) or backend::mi355_cos / mi355_sin (real-to-real).
+ *        backend::mi355 (complex) or backend::mi355_cos / mi355_sin / mi355_cos1 (real-to-real).
  */
 template <typename backend_tag, typename index = int> class fft3d {
     static_assert(backend::is_enabled<backend_tag>::value, "The requested backend is invalid or has not been enabled.");
@@ -268,6 +273,9 @@ template <typename backend_tag, typename index = int> class fft3d {
         detail::check(heffte_mi355_set_stream(p, stream));
         const double n = (double)inbox.count();
         scale_factor_ = r2r ? 1.0 / (64.0 * n) : 1.0 / n;
+        if (detail::plan_kind<backend_tag>::value == HMI_KIND_COS1)  // include/heffte_fft3d.h:643-646
+            scale_factor_ = 1.0 / (64.0 * (double)(inbox.size[0] - 1) * (double)(inbox.size[1] - 1) *
+                                   (double)(inbox.size[2] - 1));
     }
 #ifdef HEFFTE_MI355_HAVE_MPI
     //! The reference constructor; the communicator must have exactly one rank.

@@ -385,12 +385,44 @@ int orc_fft3d_c2r(const orc_box *inbox, const orc_box *outbox, int r2c_dir,
 
 /* ------------------------------------------------------------------ r2r */
 
+/* Length of the extended signal of one r2r line: 4N for the DCT-II/DST-II
+ * pair, 4(N-1) for DCT-I (cos1_pre_pos_processor::compute_extended_length,
+ * include/heffte_backend_rocm.h:165-181). */
+static int r2r_ext_len(int kind, int n) { return kind == 2 ? 4 * (n - 1) : 4 * n; }
+
+/* One DCT-I line (kind 2) as the rocFFT backend computes it, through the
+ * extended signal of length L = 4(N-1):
+ *   forward  src/heffte_backend_rocm.hip.cpp:269-297: real (x0 0 x1 0 .. x_{N-1} 0 .. x1 0),
+ *            y[k] = Re FFT[k] (FFTW's REDFT00);
+ *   backward src/heffte_backend_rocm.hip.cpp:299-328: half spectrum c[j] = x[j],
+ *            c[2(N-1)-j] = x[j], real inverse FFT, y[i] = r[2i] (2 x REDFT00). */
+static void cos1_line(const fft_plan *pl, int sign, int n, const double *x, double *y,
+                      cplx *buf, cplx *res, cplx *scratch) {
+    int L = 4 * (n - 1);
+    for (int i = 0; i < L; i++) { buf[i].re = 0.0; buf[i].im = 0.0; }
+    if (sign < 0) {
+        for (int i = 0; i < n; i++) buf[2 * i].re = x[i];
+        for (int i = 1; i < n; i++) buf[L - 2 * i].re = x[i];
+        plan_exec(pl, buf, 1, res, scratch);
+        for (int i = 0; i < n; i++) y[i] = res[i].re;
+    } else {
+        for (int j = 0; j < n; j++) { buf[j].re = x[j]; buf[2 * (n - 1) - j].re = x[j]; }
+        for (int i = 2 * (n - 1) + 1; i < L; i++) { buf[i].re = buf[L - i].re; buf[i].im = -buf[L - i].im; }
+        plan_exec(pl, buf, 1, res, scratch);
+        for (int i = 0; i < n; i++) y[i] = res[2 * i].re;
+    }
+}
+
 /* One DCT/DST line through the extended 4N signal exactly as
  * include/heffte_r2r_executor.h:43-134 defines it (pre-process, length-4N
- * real FFT, post-process).  kind 0 = cos, 1 = sin. */
+ * real FFT, post-process).  kind 0 = cos, 1 = sin, 2 = cos1 (DCT-I). */
 static void r2r_line(const fft_plan *p4, int kind, int sign, int n, const double *x, double *y,
                      cplx *buf, cplx *res, cplx *scratch) {
     int L = 4 * n;
+    if (kind == 2) {
+        cos1_line(p4, sign, n, x, y, buf, res, scratch);
+        return;
+    }
     if (sign < 0) {
         /* forward: real extended signal, FFT, post */
         for (int i = 0; i < L; i++) { buf[i].re = 0.0; buf[i].im = 0.0; }
@@ -433,9 +465,9 @@ static void r2r_line(const fft_plan *p4, int kind, int sign, int n, const double
 }
 
 void orc_r2r_1d(int kind, int sign, int n, const double *x, double *y) {
-    if (n <= 0) return;
+    if (n <= 0 || (kind == 2 && n < 2)) return;
     fft_plan p4;
-    plan_init(&p4, 4 * n, sign);
+    plan_init(&p4, r2r_ext_len(kind, n), sign);
     cplx *buf = (cplx *)malloc(sizeof(cplx) * (size_t)(4 * n + 1));
     cplx *res = (cplx *)malloc(sizeof(cplx) * (size_t)(4 * n));
     cplx *scratch = (cplx *)malloc(sizeof(cplx) * (size_t)max_factor(&p4));
@@ -449,6 +481,7 @@ static int r2r_common(int kind, const orc_box *inbox, const orc_box *outbox, int
     if (!same_indexes(inbox, outbox)) return -1;
     long long size[3], si[3], so[3];
     box_sizes(inbox, size);
+    if (kind == 2 && (size[0] < 2 || size[1] < 2 || size[2] < 2)) return -1;
     orc_box_strides(inbox, si);
     orc_box_strides(outbox, so);
     long long cnt = orc_box_count(inbox);
@@ -466,7 +499,7 @@ static int r2r_common(int kind, const orc_box *inbox, const orc_box *outbox, int
             int o2 = (axis == 2) ? 1 : 2;
             long long nlines = size[o1] * size[o2];
             fft_plan p4;
-            plan_init(&p4, 4 * n, sign);
+            plan_init(&p4, r2r_ext_len(kind, n), sign);
             int mf = max_factor(&p4);
 #pragma omp parallel
             {
@@ -505,4 +538,9 @@ int orc_r2r_cos(const orc_box *inbox, const orc_box *outbox, int sign,
 int orc_r2r_sin(const orc_box *inbox, const orc_box *outbox, int sign,
                 double scale, int batch, const double *in, double *out) {
     return r2r_common(1, inbox, outbox, sign, scale, batch, in, out);
+}
+
+int orc_r2r_cos1(const orc_box *inbox, const orc_box *outbox, int sign,
+                 double scale, int batch, const double *in, double *out) {
+    return r2r_common(2, inbox, outbox, sign, scale, batch, in, out);
 }

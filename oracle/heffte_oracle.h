@@ -86,13 +86,19 @@ int orc_fft3d_c2r(const orc_box *inbox, const orc_box *outbox, int r2c_dir,
 int orc_r2r_cos(const orc_box *inbox, const orc_box *outbox, int sign,
                 double scale, int batch, const double *in, double *out);
 
-/* One line of the r2r transform (kind 0 = cos, 1 = sin) of length n, as the
+/* One line of the r2r transform (kind 0 = cos, 1 = sin, 2 = cos1) of length n, as the
  * 1D executor real2real_executor applies it (include/heffte_r2r_executor.h:221-245). */
 void orc_r2r_1d(int kind, int sign, int n, const double *x, double *y);
 
 /* Same for the sine transform (DST-II forward, 2*DST-III backward). */
 int orc_r2r_sin(const orc_box *inbox, const orc_box *outbox, int sign,
                 double scale, int batch, const double *in, double *out);
+
+/* rtransform<rocfft_cos1>: DCT-I (REDFT00) along every axis on forward, 2 x
+ * REDFT00 on backward (src/heffte_backend_rocm.hip.cpp:269-328); every extent
+ * must be at least 2 (returns -1 otherwise). */
+int orc_r2r_cos1(const orc_box *inbox, const orc_box *outbox, int sign,
+                 double scale, int batch, const double *in, double *out);
 
 #ifdef __cplusplus
 }

@@ -71,9 +71,10 @@ def lib():
         L.orc_fft3d_c2r.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
         L.orc_r2r_cos.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
         L.orc_r2r_sin.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
+        L.orc_r2r_cos1.argtypes = [P, P, ctypes.c_int, ctypes.c_double, ctypes.c_int, dp, dp]
         L.orc_r2r_1d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]
         for f in ("orc_transpose", "orc_fft3d_c2c", "orc_fft3d_real_c2c", "orc_fft3d_r2c",
-                  "orc_fft3d_c2r", "orc_r2r_cos", "orc_r2r_sin"):
+                  "orc_fft3d_c2r", "orc_r2r_cos", "orc_r2r_sin", "orc_r2r_cos1"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -155,8 +156,16 @@ def r2r_sin(inbox, outbox, x, sign=-1, scale=1.0, batch=1):
     return y
 
 
+def r2r_cos1(inbox, outbox, x, sign=-1, scale=1.0, batch=1):
+    """DCT-I (REDFT00) forward, 2 x REDFT00 backward; every extent >= 2."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty(batch * outbox.count(), dtype=np.float64)
+    _check(lib().orc_r2r_cos1(ctypes.byref(inbox), ctypes.byref(outbox), sign, scale, batch, _ptr(x), _ptr(y)))
+    return y
+
+
 def r2r_1d(x, kind=0, sign=-1):
-    """kind 0 = cos (DCT-II fwd / 2 DCT-III bwd), 1 = sin."""
+    """kind 0 = cos (DCT-II fwd / 2 DCT-III bwd), 1 = sin, 2 = cos1 (REDFT00 fwd / 2 REDFT00 bwd)."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty_like(x)
     lib().orc_r2r_1d(kind, sign, x.size, _ptr(x), _ptr(y))

@@ -6,7 +6,7 @@ Sources (paths relative to the heFFTe 2.4.1 tree):
       test/test_units_nompi.cpp:92-97 (input), :136-202 (make_fft0/1/2, *_r2c),
       :209-262 and :336-367 (how they are applied: one executor per axis)
   * 1D DCT-II / DST-II with batch replication: test/test_units_nompi.cpp:288-333
-  * 3D DCT-II / DST-II of iota(1..24) on the 2x3x4 world: test/test_cos.cpp:36-54
+  * 3D DCT-II / DST-II / DCT-I of iota(1..24) on the 2x3x4 world: test/test_cos.cpp:36-54
   * in-node transposes of the 2x3x4 box: test/test_units_nompi.cpp:577-637
 
 Complex numbers are stored as [re, im] pairs.  Run from any directory:
@@ -108,6 +108,9 @@ data = {
                 -1.9200000000000003e+02, 0.0, 3.0614674589207186e+02, -1.7318275204678301e+01,
                 -4.2420937476555700e+01, 0.0, 1.5307337294603599e+02, -8.6591376023391504e+00,
                 -2.7152900397563417e+02, 0.0, 0.0, 0.0, -1.3576450198781720e+02, 0.0],
+        # test/test_cos.cpp:48-50 (rocfft_cos1 / stock_cos1: DCT-I), same world and input
+        "cos1": [600.0, -24.0, -48.0, 0.0, 0.0, 0.0, -192.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+                 0.0, -48.0, 0.0, 0.0, 0.0, 0.0, 0.0],
     },
     "transpose": {
         # test/test_units_nompi.cpp:595-636; source box order (0,1,2), payload 1..24

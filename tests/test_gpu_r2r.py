@@ -1,10 +1,12 @@
-"""GPU parity of rtransform<cos / sin> (DCT-II/III, DST-II/III on every axis).
+"""GPU parity of rtransform<cos / sin / cos1> (DCT-II/III, DST-II/III, DCT-I on every axis).
 
 Known answers: test/test_cos.cpp:36-80 (2x3x4 world, iota input; forward golden, backward
 with scale::full recovers the input) and test/test_units_nompi.cpp:288-333 (1D DCT-II/DST-II of
 1..4 and 1..5, embedded here in boxes whose other two axes have length 1, where each length-1
 axis multiplies by 2).  Random inputs are compared with the FP64 oracle for every order pair,
-odd/prime/power-of-two lengths, batches and in-place calls.
+odd/prime/power-of-two lengths, batches and in-place calls.  DCT-I: the golden of
+test/test_cos.cpp:48-50 and the oracle (REDFT00 forward, 2 REDFT00 backward, scale::full =
+1/(64 prod(n-1))).
 """
 import numpy as np
 import pytest
@@ -22,7 +24,7 @@ def run_r2r(hmi, cuda, kind, size, inorder, outorder, x, dtype, direction="forwa
     rdt, _ = RDT[dtype]
     inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], inorder)
     outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], outorder)
-    plan = hmi.rtransform(hmi.backend.mi355, inbox, outbox, hmi.kind.cos if kind == "cos" else hmi.kind.sin)
+    plan = hmi.rtransform(hmi.backend.mi355, inbox, outbox, getattr(hmi.kind, kind))
     tx = torch.from_numpy(np.ascontiguousarray(x).astype(rdt)).to(cuda)
     ty = tx if in_place else torch.empty_like(tx)
     ws = torch.empty(max(1, batch * plan.size_workspace()), dtype=tx.dtype, device=cuda) if use_ws else None
@@ -34,7 +36,7 @@ def run_r2r(hmi, cuda, kind, size, inorder, outorder, x, dtype, direction="forwa
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-@pytest.mark.parametrize("kind", ["cos", "sin"])
+@pytest.mark.parametrize("kind", ["cos", "sin", "cos1"])
 def test_known_answers_3d(hmi, cuda, golden, dtype, kind):
     x = np.arange(1.0, 25.0)
     y, plan = run_r2r(hmi, cuda, kind, (2, 3, 4), (0, 1, 2), (0, 1, 2), x, dtype)
@@ -95,3 +97,28 @@ def test_batch_symmetric_scale(hmi, cuda, oracle, dtype):
     bo = oracle.Box.make((0, 0, 0), [s - 1 for s in size], (2, 1, 0))
     ref = oracle.r2r_cos(bi, bo, x, -1, 1.0 / np.sqrt(64.0 * n), batch)
     assert fc.rel_l2(y, ref) <= tol
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("size", [(9, 17, 5), (33, 65, 3), (257, 5, 6), (12, 7, 2), (1025, 3, 2), (129, 129, 9)])
+def test_cos1_vs_oracle(hmi, cuda, oracle, dtype, size):
+    """DCT-I: kernel FFT lengths 2(n-1) from 2 to 2048, compile-time and runtime-plan passes."""
+    rdt, tol = RDT[dtype]
+    rng = np.random.default_rng(sum(size) + 1)
+    for inorder, outorder, in_place in [((0, 1, 2), (0, 1, 2), False), ((0, 1, 2), (0, 1, 2), True),
+                                        ((2, 0, 1), (1, 2, 0), False)]:
+        n = int(np.prod(size))
+        x = rng.standard_normal(n)
+        y, plan = run_r2r(hmi, cuda, "cos1", size, inorder, outorder, x, dtype, in_place=in_place)
+        bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size], inorder)
+        bo = oracle.Box.make((0, 0, 0), [s - 1 for s in size], outorder)
+        assert fc.rel_l2(y, oracle.r2r_cos1(bi, bo, x, -1)) <= tol, (inorder, outorder, in_place)
+        z, _ = run_r2r(hmi, cuda, "cos1", size, inorder, outorder, y, dtype, "backward", hmi.scale.full,
+                       in_place=in_place)
+        assert fc.rel_l2(z, x) <= tol, (inorder, outorder, in_place)
+
+
+def test_cos1_rejects_single_point_axis(hmi):
+    b = hmi.box3d([0, 0, 0], [7, 0, 7])
+    with pytest.raises(Exception):
+        hmi.rtransform(hmi.backend.mi355, b, b, hmi.kind.cos1)

@@ -82,15 +82,16 @@ def test_1d_r2r_known_answers(oracle, golden, name, kind):
     assert np.abs(back - x).max() < 1e-12
 
 
-@pytest.mark.parametrize("kind", ["cos", "sin"])
+@pytest.mark.parametrize("kind", ["cos", "sin", "cos1"])
 def test_3d_r2r_known_answers(oracle, golden, kind):
-    """test/test_cos.cpp:36-80: forward golden, backward with scale::full = 1/(64 N)."""
+    """test/test_cos.cpp:36-80: forward golden, backward with scale::full = 1/(64 N)
+    (DCT-I: 1/(64 (n0-1)(n1-1)(n2-1)), include/heffte_fft3d.h:643-646)."""
     b = box(oracle, (0, 0, 0), (1, 2, 3))
     x = np.arange(1.0, 25.0)
-    fn = oracle.r2r_cos if kind == "cos" else oracle.r2r_sin
+    fn = {"cos": oracle.r2r_cos, "sin": oracle.r2r_sin, "cos1": oracle.r2r_cos1}[kind]
     y = fn(b, b, x, -1)
     assert np.abs(y - np.array(golden["r2r_3d"][kind])).max() < 1e-9
-    back = fn(b, b, y, +1, 1.0 / (64 * 24))
+    back = fn(b, b, y, +1, 1.0 / (64 * 24) if kind != "cos1" else 1.0 / (64 * 1 * 2 * 3))
     assert np.abs(back - x).max() < 1e-11
 
 
@@ -236,6 +237,40 @@ def test_r2r_1d_closed_forms(oracle, n):
     assert np.abs(oracle.r2r_1d(x, 0, +1) - dct3x2_closed(x)).max() < 1e-12 * n
     assert np.abs(oracle.r2r_1d(x, 1, -1) - dst2_closed(x)).max() < 1e-12 * n
     assert np.abs(oracle.r2r_1d(x, 1, +1) - dst3x2_closed(x)).max() < 1e-12 * n
+
+
+def redft00_closed(x):
+    """DCT-I: y[k] = x[0] + (-1)^k x[N-1] + 2 sum_{j=1}^{N-2} x[j] cos(pi j k / (N-1))."""
+    n = x.size
+    k = np.arange(n)[:, None]
+    j = np.arange(1, n - 1)[None, :]
+    return x[0] + (-1.0) ** np.arange(n) * x[-1] + 2.0 * (np.cos(np.pi * j * k / (n - 1)) @ x[1:-1])
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 9, 17, 33, 65])
+def test_cos1_1d_closed_form(oracle, n):
+    """DCT-I forward = REDFT00, backward = 2 REDFT00; round trip 4 (N-1)."""
+    x = np.random.default_rng(n + 7).standard_normal(n)
+    y = oracle.r2r_1d(x, 2, -1)
+    assert np.abs(y - redft00_closed(x)).max() < 1e-12 * n
+    assert np.abs(oracle.r2r_1d(x, 2, +1) - 2.0 * redft00_closed(x)).max() < 1e-12 * n
+    assert np.abs(oracle.r2r_1d(y, 2, +1) / (4 * (n - 1)) - x).max() < 1e-13 * n
+
+
+def test_cos1_3d_separable_and_rejects_short_axes(oracle):
+    rng = np.random.default_rng(19)
+    size = (5, 3, 9)
+    a = rng.standard_normal(size)
+    ref = np.apply_along_axis(redft00_closed, 0, a)
+    ref = np.apply_along_axis(redft00_closed, 1, ref)
+    ref = np.apply_along_axis(redft00_closed, 2, ref)
+    bi = box(oracle, (0, 0, 0), (4, 2, 8), (1, 2, 0))
+    bo = box(oracle, (0, 0, 0), (4, 2, 8), (2, 0, 1))
+    y = oracle.r2r_cos1(bi, bo, flat(a, (1, 2, 0)))
+    assert oracle.rel_l2(y, flat(ref, (2, 0, 1))) < TOL64
+    b1 = box(oracle, (0, 0, 0), (4, 0, 8))
+    with pytest.raises(Exception):
+        oracle.r2r_cos1(b1, b1, np.zeros(45))
 
 
 def test_r2r_3d_separable(oracle):

@@ -302,6 +302,51 @@ def rtransform(backend_tag, inbox, outbox, transform=kind.cos, comm=None):
     return fft_plan(h, transform)
 
 
+def proc_setup(world, num_procs):
+    """Processor grid (i, j, k), i j k = num_procs, whose sub-boxes have the least surface
+    (python/heffte.py:428-448 with the integer semantics of proc_setup_min_surface,
+    include/heffte_geometry.h:643-686).  This backend runs one rank: proc_setup(world, 1)
+    is (1, 1, 1); larger grids only describe how a caller would split the world."""
+    if world.count() <= 0 or num_procs < 1:
+        raise heffte_input_error("proc_setup() needs a non-empty world and at least one rank")
+    if num_procs == 1:
+        return np.array([1, 1, 1], dtype=np.int32)
+    n = world.size
+
+    def surface(g):
+        b = [n[d] // g[d] for d in range(3)]
+        return b[0] * b[1] + b[1] * b[2] + b[2] * b[0]
+
+    best, best_surface = [1, 1, 1], None
+    for i in range(1, min(num_procs, n[0]) + 1):
+        if num_procs % i:
+            continue
+        jmax = min(num_procs // i, n[1])
+        for j in range(1, jmax + 1):
+            if jmax % j:
+                continue
+            k = num_procs // (i * j)
+            if k <= n[2] and i * j * k == num_procs:
+                s = surface((i, j, k))
+                if best_surface is None or s < best_surface:
+                    best, best_surface = [i, j, k], s
+    return np.array(best, dtype=np.int32)
+
+
+def split_world(world, proc_grid):
+    """heffte::split_world (include/heffte_geometry.h:410-431): the boxes of a processor grid,
+    fast index first; the first (size % grid) boxes along an axis get one extra index."""
+    g = [int(v) for v in proc_grid]
+    n = world.size
+
+    def edge(d, i):
+        return int(world.low[d]) + i * (n[d] // g[d]) + min(i, n[d] % g[d])
+
+    return [box3d([edge(0, i), edge(1, j), edge(2, k)], [edge(0, i + 1) - 1, edge(1, j + 1) - 1, edge(2, k + 1) - 1],
+                  world.order)
+            for k in range(g[2]) for j in range(g[1]) for i in range(g[0])]
+
+
 def reshape(inbox, out_order, inarray, outarray, batch=1, stream=None):
     """Bit-exact in-node reshape of inarray (stored in inbox.order) into out_order."""
     il, ih, io = inbox._c()

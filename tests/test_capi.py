@@ -109,3 +109,39 @@ def test_default_options(hmi):
     o = opts()
     assert hmi.lib.heffte_mi355_set_default_options(ctypes.byref(o)) == 0
     assert o.use_reorder == 0 and o.use_pencils == 1
+
+
+def test_cos1_plan_rules(hmi):
+    """DCT-I plans need two points per axis; the workspace covers the real intermediates."""
+    b = hmi.box3d([0, 0, 0], [8, 16, 4])
+    p = hmi.rtransform(hmi.backend.mi355, b, b, hmi.kind.cos1)
+    assert p.size_inbox() == p.size_outbox() == 9 * 17 * 5
+    assert p.size_workspace() >= 0
+    with pytest.raises(hmi.heffte_input_error):
+        flat = hmi.box3d([0, 0, 0], [8, 0, 4])
+        hmi.rtransform(hmi.backend.mi355, flat, flat, hmi.kind.cos1)
+
+
+def test_proc_setup_and_split_world(hmi):
+    """Host geometry helpers (python/heffte.py:428-460): one rank owns the world; larger grids
+    partition it exactly, with the remainder on the first boxes."""
+    world = hmi.box3d([0, 0, 0], [511, 511, 511])
+    assert list(hmi.proc_setup(world, 1)) == [1, 1, 1]
+    assert sorted(hmi.proc_setup(world, 8).tolist()) == [2, 2, 2]
+    assert list(hmi.proc_setup(hmi.box3d([0, 0, 0], [1023, 1023, 3]), 4)) == [2, 2, 1]
+    one = hmi.split_world(world, [1, 1, 1])
+    assert len(one) == 1 and one[0].size == world.size
+    w2 = hmi.box3d([2, 0, 1], [11, 6, 5], (2, 0, 1))
+    parts = hmi.split_world(w2, [3, 2, 1])
+    assert len(parts) == 6 and sum(p.count() for p in parts) == w2.count()
+    assert parts[0].size == (4, 4, 5) and parts[1].size == (3, 4, 5) and parts[3].size == (4, 3, 5)
+    assert parts[0].low.tolist() == [2, 0, 1] and list(parts[5].high) == [11, 6, 5]
+    assert all(list(p.order) == [2, 0, 1] for p in parts)
+    covered = set()
+    for p in parts:
+        for i in range(p.low[0], p.high[0] + 1):
+            for j in range(p.low[1], p.high[1] + 1):
+                for k in range(p.low[2], p.high[2] + 1):
+                    assert (i, j, k) not in covered
+                    covered.add((i, j, k))
+    assert len(covered) == w2.count()

@@ -41,6 +41,8 @@ def parse():
     p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     p.add_argument("--no-batched", action="store_true", help="skip the 1000-transform mixed-size batch leg")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA forms")
+    p.add_argument("--no-secondary", action="store_true",
+                   help="skip the r2c 1024x1024x512 and DCT 256^3 legs")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return p.parse_args()
@@ -131,6 +133,97 @@ def batched_leg(heffte, torch, dev, steps):
             "ms_per_pair_batched_graph": round(ms_batched, 3), "ms_per_pair_launch_loop": round(ms_loop, 3),
             "speedup": round(ms_loop / ms_batched, 2),
             "round_trip_rel_l2": max(err_b, err_l)}
+
+
+def _timed(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def plane_waves(torch, dev, shape, seed, dtype):
+    """16 plane-wave cosines (integer wave vectors in [0, 32)^3, amplitudes U[0, 1]) plus
+    1e-3 N(0, 1) noise on a grid of shape (n0, n1, n2), x fastest."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n0, n1, n2 = shape
+    i = torch.arange(n0, device=dev, dtype=torch.float64) / n0
+    j = torch.arange(n1, device=dev, dtype=torch.float64) / n1
+    k = torch.arange(n2, device=dev, dtype=torch.float64) / n2
+    x = torch.zeros((n2, n1, n0), device=dev, dtype=dtype)
+    for _ in range(16):
+        kv = rng.integers(0, 32, 3)
+        amp = float(rng.uniform())
+        ph = (2 * math.pi * (kv[2] * k[:, None, None] + kv[1] * j[None, :, None])).to(dtype)
+        x += amp * torch.cos(ph + (2 * math.pi * kv[0] * i[None, None, :]).to(dtype))
+        del ph
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    x += 1e-3 * torch.randn(x.shape, device=dev, dtype=dtype, generator=g)
+    return x.reshape(-1)
+
+
+def r2c_leg(heffte, torch, dev, steps):
+    """r2c along axis 0 of a 1024 x 1024 x 512 float box, in and out order (0,1,2), out of
+    place: one real-data pass (2N reals -> N/2+1 bins) and two complex passes."""
+    n0, n1, n2 = 1024, 1024, 512
+    inbox = heffte.box3d([0, 0, 0], [n0 - 1, n1 - 1, n2 - 1])
+    outbox = inbox.r2c(0)
+    plan = heffte.fft3d_r2c(heffte.backend.mi355, inbox, outbox, 0)
+    x = plane_waves(torch, dev, (n0, n1, n2), 2, torch.float32)
+    y = torch.empty(outbox.count(), dtype=torch.complex64, device=dev)
+    z = torch.empty_like(x)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=torch.complex64, device=dev)
+    ms_f = _timed(torch, lambda: plan.forward_buffered(x, y, ws), steps)
+    ms_b = _timed(torch, lambda: plan.backward_buffered(y, z, ws, heffte.scale.full), steps)
+    nr, nc = inbox.count(), outbox.count()
+    bytes_f = 4 * nr + 8 * nc + 2 * 2 * 8 * nc
+    rt = (torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x)).item()
+    del z, ws
+    x64 = x.to(torch.float64)
+    del x
+    ws64 = torch.empty(max(1, plan.size_workspace()), dtype=torch.complex128, device=dev)
+    y64 = torch.empty(nc, dtype=torch.complex128, device=dev)
+    plan.forward_buffered(x64, y64, ws64)
+    ms_f64 = _timed(torch, lambda: plan.forward_buffered(x64, y64, ws64), max(2, steps // 2))
+    err = (torch.linalg.vector_norm(y.to(torch.complex128) - y64) / torch.linalg.vector_norm(y64)).item()
+    del x64, y64, ws64, y
+    torch.cuda.empty_cache()
+    return {"workload": "r2c 1024x1024x512 float -> 513x1024x512 complex<float>, axis 0, order (0,1,2), "
+                        "out of place, 16 plane waves + 1e-3 noise",
+            "ms_forward": round(ms_f, 4), "ms_backward": round(ms_b, 4),
+            "forward_hbm_GBps": round(bytes_f / (ms_f * 1e-3) / 1e9, 1),
+            "forward_hbm_frac": round(bytes_f / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "forward_bytes": bytes_f, "ms_forward_double": round(ms_f64, 4),
+            "rel_l2_vs_fp64": err, "round_trip_rel_l2": rt}
+
+
+def dct_leg(heffte, torch, dev, steps):
+    """rtransform<cos> (DCT-II forward, DCT-III backward) of a 256^3 double box, order (0,1,2),
+    out of place: three real-to-real passes per direction."""
+    n = 256
+    box = heffte.box3d([0, 0, 0], [n - 1] * 3)
+    plan = heffte.rtransform(heffte.backend.mi355, box, box, heffte.kind.cos)
+    x = plane_waves(torch, dev, (n, n, n), 2, torch.float64)
+    y, z = torch.empty_like(x), torch.empty_like(x)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=torch.float64, device=dev)
+    ms_f = _timed(torch, lambda: plan.forward_buffered(x, y, ws), steps)
+    ms_b = _timed(torch, lambda: plan.backward_buffered(y, z, ws, heffte.scale.full), steps)
+    bytes_f = 3 * 2 * 8 * n ** 3
+    rt = (torch.linalg.vector_norm(z - x) / torch.linalg.vector_norm(x)).item()
+    del x, y, z, ws
+    torch.cuda.empty_cache()
+    return {"workload": "DCT-II/III 256^3 double, order (0,1,2), out of place, 16 plane waves + 1e-3 noise",
+            "ms_forward": round(ms_f, 4), "ms_backward": round(ms_b, 4),
+            "forward_hbm_GBps": round(bytes_f / (ms_f * 1e-3) / 1e9, 1),
+            "forward_hbm_frac": round(bytes_f / (ms_f * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "forward_bytes": bytes_f, "round_trip_rel_l2": rt}
 
 
 def main():
@@ -292,6 +385,13 @@ def main():
         torch.cuda.empty_cache()
         out["batched_mixed_1000"] = batched_leg(heffte, torch, dev, max(3, args.steps // 4))
         torch.cuda.empty_cache()
+        data = ws = None
+
+    if not args.no_secondary and n == 512:
+        del data, ws
+        torch.cuda.empty_cache()
+        out["r2c_1024x1024x512_f32"] = r2c_leg(heffte, torch, dev, max(3, args.steps // 2))
+        out["dct_256_f64"] = dct_leg(heffte, torch, dev, max(3, args.steps // 2))
         data = ws = None
 
     if not args.no_cpu:

@@ -210,6 +210,22 @@ template <typename R> void test_r2r(double tol) {
     for (int i = 0; i < 24; i++) worst = std::max(worst, std::abs((double)hs[i] - (double)x[i]));
     std::snprintf(name, 128, "%s rtransform sin round trip", pn);
     CHECK(worst < (sizeof(R) == 4 ? 1e-3 : 1e-11), name);
+    // DCT-I, test/test_cos.cpp:48-50; scale::full = 1 / (64 (n0-1)(n1-1)(n2-1))
+    std::vector<double> cos1ref(24, 0.0);
+    cos1ref[0] = 600.0; cos1ref[1] = -24.0; cos1ref[2] = -48.0; cos1ref[6] = -192.0; cos1ref[18] = -48.0;
+    heffte::rtransform<heffte::backend::mi355_cos1> tcos1(world, world);
+    auto g1 = tcos1.forward(gx);
+    auto h1 = heffte::gpu::transfer::unload(g1);
+    worst = 0;
+    for (int i = 0; i < 24; i++) worst = std::max(worst, std::abs((double)h1[i] - cos1ref[i]));
+    std::snprintf(name, 128, "%s rtransform cos1 known answer", pn);
+    CHECK(worst < (sizeof(R) == 4 ? 1e-3 : 1e-10), name);
+    CHECK(std::abs(tcos1.get_scale_factor(heffte::scale::full) - 1.0 / 384.0) < 1e-15, "cos1 scale factor");
+    auto h1b = heffte::gpu::transfer::unload(tcos1.backward(g1, heffte::scale::full));
+    worst = 0;
+    for (int i = 0; i < 24; i++) worst = std::max(worst, std::abs((double)h1b[i] - (double)x[i]));
+    std::snprintf(name, 128, "%s rtransform cos1 round trip", pn);
+    CHECK(worst < (sizeof(R) == 4 ? 1e-3 : 1e-11), name);
     (void)tol;
 }
 

@@ -16,6 +16,7 @@
 // element (i % 4) + 4 (i / 8).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -250,6 +251,195 @@ fft512_mfma(pass_args a) {
     }
 }
 
+// Persistent form: one 1024-thread block per CU walks the tiles blockIdx.x, blockIdx.x +
+// gridDim.x, ...  The split DFT operands of both GEMMs and W512 are copied to LDS once, and
+// the global loads of the next tile are in flight while the current tile runs step A.  Both
+// tile layouts, the tables and the per-line maxima (by tile parity: one partial per wave and
+// line) take 163,728 B of LDS: one block per CU.
+// Per tile: issue loads(next) | step A | barrier | Y -> LDS, next -> LDS | barrier | step C, store.
+template <int PREC>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+fft512_mfma_persistent(pass_args a, unsigned ntiles) {
+    constexpr int T = 16, P = 64, E = 8;
+    constexpr int RA = 33, RLA = 16 * RA + 1;
+    constexpr int RC = 17, RLC = 32 * RC + 1;
+    using S = split_traits<PREC>;
+    using vec = typename S::vec;
+    constexpr int TERMS = S::terms;
+    __shared__ float2 ldsA[T * RLA];
+    __shared__ float2 ldsC[T * RLC];
+    __shared__ float2 w512s[466];  // W512^r for r = n2 k1 <= 15 * 31
+    __shared__ vec opAs[2 * 4 * TERMS * 64];
+    __shared__ vec opCs[2 * TERMS * 64];
+    __shared__ float pmax[2][T][T];
+
+    const float sg = a.conj ? -1.0f : 1.0f;
+    const unsigned nlines = (unsigned)a.nlines;
+    const bool lcf = a.load_colfast != 0, scf = a.store_colfast != 0;
+    {
+        const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
+        const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);
+        const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;
+        const int tid = threadIdx.x;
+        for (int i = tid; i < 2 * 4 * TERMS * 64; i += 1024) opAs[i] = opA[i];
+        if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
+        if (tid < 466) w512s[tid] = w512[tid];
+    }
+    float *ldsCf = reinterpret_cast<float *>(ldsC);
+    const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
+    const long long in_step = (long long)P * a.in_sa;
+
+    // The thread index is passed through an empty asm at each use so that the per-lane
+    // addresses derived from it are recomputed in the loop instead of being kept live
+    // across it (128 VGPRs at this occupancy).
+    auto thread = [] {
+        int v = threadIdx.x;
+        asm volatile("" : "+v"(v));
+        return v;
+    };
+    // all eight loads of a tile in flight (an invalid line reads line 0)
+    auto issue = [&](unsigned tile, float2 (&x)[E]) {
+        const int tid = thread();
+        const int l = lcf ? (tid & 15) : (tid >> 6), t = lcf ? (tid >> 4) : (tid & 63);
+        const unsigned L = tile * T + l;
+        const float2 *p = src + (L < nlines ? line_off(L, a, false) : 0) + (long long)t * a.in_sa;
+#pragma unroll
+        for (int e = 0; e < E; e++) x[e] = p[e * in_step];
+    };
+    // sign, step-A layout (x[16 n1 + n2] at row n2, column n1; pos = t + 64 e has
+    // n2 = t & 15, n1 = (t >> 4) + 4 e), per-line maximum: a full line per wave (along the axis)
+    // or lanes l, l+16, l+32, l+48 of every wave (column-fast), one partial per wave
+    auto deposit = [&](unsigned tile, float2 (&x)[E], int par) {
+        const int tid = thread();
+        const int lane = tid & 63, wave = tid >> 6;
+        const int l = lcf ? (tid & 15) : (tid >> 6), t = lcf ? (tid >> 4) : (tid & 63);
+        const bool valid = tile * T + l < nlines;
+        float2 *row = &ldsA[l * RLA + (t & 15) * RA + (t >> 4)];
+        float m = 0.f;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            if (!valid) x[e] = make_float2(0.f, 0.f);
+            x[e].y *= sg;
+            m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
+            row[4 * e] = x[e];
+        }
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        if (!lcf) {
+#pragma unroll
+            for (int off = 8; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if (lane == 0) pmax[par][l][0] = m;
+        } else if (lane < 16) {
+            pmax[par][l][wave] = m;
+        }
+    };
+    auto line_max = [&](int par, int line) {
+        if (!lcf) return pmax[par][line][0];
+        float m = 0.f;
+#pragma unroll
+        for (int w = 0; w < T; w++) m = fmaxf(m, pmax[par][line][w]);
+        return m;
+    };
+
+    unsigned tile = blockIdx.x;
+    {
+        float2 x[E];
+        issue(tile, x);
+        deposit(tile, x, 0);
+    }
+    __syncthreads();
+    for (int it = 0; tile < ntiles; it++, tile += gridDim.x) {
+        const int par = it & 1;
+        const unsigned next = tile + gridDim.x;
+        const bool has_next = next < ntiles;
+        float2 xn[E];
+        if (has_next) issue(next, xn);
+
+        // ---- step A: wave -> (column block b, real / imaginary rows m); column j = (n2, line)
+        f32x16_t acc = {};
+        {
+            const int tid = thread();
+            const int lane = tid & 63, wave = tid >> 6, j = lane & 31, h = lane >> 5;
+            const int m = wave & 1, b = wave >> 1;
+            const int n2 = j & 15, line = 2 * b + (j >> 4);
+            int ex = 0;
+            (void)frexpf(line_max(par, line), &ex);
+            const float s = ldexpf(1.0f, -ex);
+            const float2 *rowp = &ldsA[line * RLA + n2 * RA + 8 * h];
+            const vec *opm = &opAs[m * 4 * TERMS * 64 + lane];
+#pragma unroll
+            for (int half = 0; half < 2; half++) {
+                float re[8], im[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const float2 x = rowp[16 * half + q];
+                    re[q] = x.x * s;
+                    im[q] = x.y * s;
+                }
+#pragma unroll
+                for (int part = 0; part < 2; part++) {
+                    const int ks = half + 2 * part;
+                    vec A[3], B[3];
+#pragma unroll
+                    for (int tt = 0; tt < TERMS; tt++) A[tt] = opm[(ks * TERMS + tt) * 64];
+                    split8<PREC>(part == 0 ? re : im, B);
+                    acc = split_mma<PREC>(A, B, acc);
+                }
+            }
+        }
+        __syncthreads();  // step A has read the tile; step C of the previous tile is done
+        {
+            const int tid = thread();
+            const int lane = tid & 63, wave = tid >> 6, j = lane & 31, h = lane >> 5;
+            const int m = wave & 1, b = wave >> 1;
+            const int n2 = j & 15, line = 2 * b + (j >> 4);
+            float *yp = &ldsCf[2 * (line * RLC + 4 * h * RC + n2) + m];
+#pragma unroll
+            for (int e = 0; e < 16; e++) yp[2 * RC * ((e & 3) + 8 * (e >> 2))] = acc[e];  // k1 = that + 4 h
+        }
+        if (has_next) deposit(next, xn, par ^ 1);
+        __syncthreads();
+
+        // ---- twiddle + step C: wave -> 32 columns (k1, line)
+        {
+            const int tid = thread();
+            const int lane = tid & 63, wave = tid >> 6, j = lane & 31, h = lane >> 5;
+            const int line = scf ? (j & 15) : wave, k1 = scf ? 2 * wave + (j >> 4) : j;
+            acc = f32x16_t{};
+            float zr[8], zi[8];
+            const float2 *yp = &ldsC[line * RLC + k1 * RC + 8 * h];
+            const float2 *wp = &w512s[8 * h * k1];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float2 y = yp[e];
+                const float2 w = wp[e * k1];  // W512^{n2 k1}, n2 = 8 h + e
+                zr[e] = y.x * w.x - y.y * w.y;
+                zi[e] = y.x * w.y + y.y * w.x;
+            }
+#pragma unroll
+            for (int ks = 0; ks < 2; ks++) {
+                vec A[3], B[3];
+#pragma unroll
+                for (int tt = 0; tt < TERMS; tt++) A[tt] = opCs[(ks * TERMS + tt) * 64 + lane];
+                split8<PREC>(ks == 0 ? zr : zi, B);
+                acc = split_mma<PREC>(A, B, acc);
+            }
+            // store: lane holds X[k1 + 32 k2], k2 = (e & 3) + 8 (e >> 2) + 4 h, e < 8
+            const unsigned L = tile * T + line;
+            if (L < nlines) {
+                int ex = 0;
+                (void)frexpf(line_max(par, line), &ex);
+                const float s = ldexpf((float)a.scale, ex);
+                float2 *__restrict__ p = static_cast<float2 *>(a.out) + line_off(L, a, true) +
+                                         (long long)(k1 + 128 * h) * a.out_sa;
+                const long long step = 32 * a.out_sa;
+#pragma unroll
+                for (int e = 0; e < 8; e++) p[((e & 3) + 8 * (e >> 2)) * step] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
+            }
+        }
+    }
+}
+
 namespace {
 long double cpi() { return 3.141592653589793238462643383279502884L; }
 // exp(-2 pi i r / m) in long double
@@ -314,9 +504,20 @@ std::vector<unsigned char> mfma512_operand_table(int prec) {
 }
 
 hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s) {
-    if (prec == kSplitF16) fft512_mfma<kSplitF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
-    else if (prec == kSplitBF16) fft512_mfma<kSplitBF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
-    else return hipErrorInvalidValue;
+    if (nblocks <= 0) return hipSuccess;
+    if (prec == kSplitF16) {
+        // one block per CU, never more blocks than tiles
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        const long long grid = std::min<long long>(nblocks, std::max(1, cus));
+        fft512_mfma_persistent<kSplitF16><<<dim3((unsigned)grid), dim3(1024), 0, s>>>(a, (unsigned)nblocks);
+    } else if (prec == kSplitBF16) {
+        fft512_mfma<kSplitBF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -11,6 +11,41 @@ template <typename R> struct cvec;
 template <> struct cvec<float> { using type = float2; };
 template <> struct cvec<double> { using type = double2; };
 
+// Streaming (non-temporal) global accesses for the pass data: every element is read once and
+// written once per pass, so the data should not displace the cached twiddle tables.
+typedef float hmi_f2v __attribute__((ext_vector_type(2)));
+typedef double hmi_d2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2 *p) {
+    const hmi_f2v v = __builtin_nontemporal_load(reinterpret_cast<const hmi_f2v *>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_stream(const double2 *p) {
+    const hmi_d2v v = __builtin_nontemporal_load(reinterpret_cast<const hmi_d2v *>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float ld_stream(const float *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ double ld_stream(const double *p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ void st_stream(float2 *p, float2 v) {
+    const hmi_f2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<hmi_f2v *>(p));
+}
+__device__ __forceinline__ void st_stream(double2 *p, double2 v) {
+    const hmi_d2v w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<hmi_d2v *>(p));
+}
+__device__ __forceinline__ void st_stream(float *p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+// NT selects the streaming form; passes whose output fits the 256 MB Infinity Cache keep the
+// default policy so the next pass can read it from there.
+template <bool NT, typename T> __device__ __forceinline__ T ldg(const T *p) {
+    if constexpr (NT) return ld_stream(p);
+    else return *p;
+}
+template <bool NT, typename T> __device__ __forceinline__ void stg(T *p, T v) {
+    if constexpr (NT) st_stream(p, v);
+    else *p = v;
+}
+
 template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) re, decltype(C::x) im) {
     C c;
     c.x = re;

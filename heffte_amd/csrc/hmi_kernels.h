@@ -10,6 +10,8 @@ namespace hmi {
 
 // Largest dynamic LDS the generic pass may request (gfx950: 160 KiB per workgroup).
 constexpr int kGenericMaxLds = 160 * 1024;
+// pass outputs above this size use non-temporal loads and stores (larger than the MALL)
+constexpr long long kStreamBytes = 256LL << 20;
 
 using fast_launcher = hipError_t (*)(const pass_args &, long long nblocks, hipStream_t);
 

@@ -95,7 +95,7 @@ __device__ __forceinline__ long long line_offset(unsigned L, const pass_args &a,
 }
 
 // One tile (T lines) of an n-point pass; the block has exactly (n / E) * T threads.
-template <typename R, typename RL, int E, int T, int PS>
+template <typename R, typename RL, int E, int T, int PS, bool NT = false>
 __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *lds) {
     using C = typename cvec<R>::type;
     constexpr int N = RL::n();
@@ -121,12 +121,21 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
         const bool valid = active && L < (unsigned)a.nlines;
         const long long base = valid ? line_offset(L, a, false) : 0;
         if (a.mode == kModeR2C) {
-            // real line of 2N values read as N complex pairs z[m] = x[2m] + i x[2m+1]
+            // real line of 2N values read as N complex pairs z[m] = x[2m] + i x[2m+1]; a contiguous
+            // line (even offsets: every other stride is a multiple of 2N) of an array aligned to
+            // the pair size is read pairwise
             const R *__restrict__ src = static_cast<const R *>(a.in);
+            if (a.in_sa == 1 && (reinterpret_cast<size_t>(a.in) & (sizeof(C) - 1)) == 0) {
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const long long m2 = 2LL * (t + P * e);
-                v[e] = valid ? cmk<C>(src[base + m2 * a.in_sa], src[base + (m2 + 1) * a.in_sa]) : cmk<C>(R(0), R(0));
+                for (int e = 0; e < E; e++)
+                    v[e] = valid ? ldg<NT>(reinterpret_cast<const C *>(src + base) + (t + P * e)) : cmk<C>(R(0), R(0));
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const long long m2 = 2LL * (t + P * e);
+                    v[e] = valid ? cmk<C>(ldg<NT>(&src[base + m2 * a.in_sa]), ldg<NT>(&src[base + (m2 + 1) * a.in_sa]))
+                                 : cmk<C>(R(0), R(0));
+                }
             }
         } else if (a.mode == kModeC2R) {
             // half spectrum X[0..N] of a real line of 2N values -> Z[k] = (X[k] + conj X[N-k])
@@ -136,8 +145,8 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
 #pragma unroll
             for (int e = 0; e < E; e++)
                 lds[l * RS + padj<PS>(t + P * e)] =
-                    valid ? src[base + (long long)(t + P * e) * a.in_sa] : cmk<C>(R(0), R(0));
-            if (t == 0) lds[l * RS + padj<PS>(N)] = valid ? src[base + (long long)N * a.in_sa] : cmk<C>(R(0), R(0));
+                    valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
+            if (t == 0) lds[l * RS + padj<PS>(N)] = valid ? ldg<NT>(&src[base + (long long)N * a.in_sa]) : cmk<C>(R(0), R(0));
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; e++) {
@@ -160,7 +169,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             for (int e = 0; e < E; e++) {
                 const int n = t + P * e;
                 const int j = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
-                R x = valid ? src[base + (long long)j * a.in_sa] : R(0);
+                R x = valid ? ldg<NT>(&src[base + (long long)j * a.in_sa]) : R(0);
                 if (sine && (j & 1)) x = -x;
                 v[e] = cmk<C>(x, R(0));
             }
@@ -172,8 +181,8 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
                 const int k = t + P * e;
                 const int jk = sine ? N - 1 - k : k;
                 const int jm = sine ? k - 1 : N - k;
-                const R xk = valid ? src[base + (long long)jk * a.in_sa] : R(0);
-                const R xm = (valid && k > 0) ? src[base + (long long)jm * a.in_sa] : R(0);
+                const R xk = valid ? ldg<NT>(&src[base + (long long)jk * a.in_sa]) : R(0);
+                const R xm = (valid && k > 0) ? ldg<NT>(&src[base + (long long)jm * a.in_sa]) : R(0);
                 const C w = tw_half[k];  // exp(-i pi k / 2N)
                 // W = conj(w) (xk - i xm); the inverse DFT runs as conj(DFT(conj W))
                 const C W = cmk<C>(w.x * xk - w.y * xm, -w.x * xm - w.y * xk);
@@ -186,27 +195,27 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             for (int e = 0; e < E; e++) {
                 const int j = t + P * e;
                 const int jj = (j <= N / 2) ? j : N - j;
-                v[e] = cmk<C>(valid ? src[base + (long long)jj * a.in_sa] : R(0), R(0));
+                v[e] = cmk<C>(valid ? ldg<NT>(&src[base + (long long)jj * a.in_sa]) : R(0), R(0));
             }
         } else if (a.in_real) {
             const R *__restrict__ src = static_cast<const R *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++)
-                v[e] = valid ? cmk<C>(src[base + (long long)(t + P * e) * a.in_sa], R(0)) : cmk<C>(R(0), R(0));
+                v[e] = valid ? cmk<C>(ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]), R(0)) : cmk<C>(R(0), R(0));
         } else if (a.herm_load) {
             const C *__restrict__ src = static_cast<const C *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int pos = t + P * e;
                 const bool mirror = pos > N / 2;
-                C x = valid ? src[base + (long long)(mirror ? N - pos : pos) * a.in_sa] : cmk<C>(R(0), R(0));
+                C x = valid ? ldg<NT>(&src[base + (long long)(mirror ? N - pos : pos) * a.in_sa]) : cmk<C>(R(0), R(0));
                 v[e] = cmk<C>(x.x, (mirror ? -sg : sg) * x.y);
             }
         } else {
             const C *__restrict__ src = static_cast<const C *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++) {
-                C x = valid ? src[base + (long long)(t + P * e) * a.in_sa] : cmk<C>(R(0), R(0));
+                C x = valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
                 v[e] = cmk<C>(x.x, sg * x.y);
             }
         }
@@ -243,8 +252,8 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const C od = cmk<C>(R(0.5) * (zk.y + zm.y), R(-0.5) * (zk.x - zm.x));
             const C w = tw_half[k];
             const C x = cadd(ev, cmul(w, od));
-            if (svalid) dst[obase + (long long)k * a.out_sa] = cmk<C>(x.x * s, x.y * s);
-            if (k == 0 && svalid) dst[obase + (long long)N * a.out_sa] = cmk<C>((ev.x - od.x) * s, (ev.y - od.y) * s);
+            if (svalid) stg<NT>(&dst[obase + (long long)k * a.out_sa], cmk<C>(x.x * s, x.y * s));
+            if (k == 0 && svalid) stg<NT>(&dst[obase + (long long)N * a.out_sa], cmk<C>((ev.x - od.x) * s, (ev.y - od.y) * s));
         }
     } else if (!svalid) {
         // column beyond the box: nothing to store
@@ -256,7 +265,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const int k = t2 + P * e;
             const C w = tw_half[k];
             const R y = R(2) * (w.x * v[e].x - w.y * v[e].y);
-            dst[obase + (long long)(sine ? N - 1 - k : k) * a.out_sa] = y * s;
+            stg<NT>(&dst[obase + (long long)(sine ? N - 1 - k : k) * a.out_sa], y * s);
         }
     } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
         const bool sine = (a.mode == kModeDST3);
@@ -267,46 +276,52 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const int j = (m < (N + 1) / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
             R y = R(2) * v[e].x;
             if (sine && (j & 1)) y = -y;
-            dst[obase + (long long)j * a.out_sa] = y * s;
+            stg<NT>(&dst[obase + (long long)j * a.out_sa], y * s);
         }
     } else if (a.mode == kModeDCT1 || a.mode == kModeDCT1B) {
         R *__restrict__ dst = static_cast<R *>(a.out);
         const R f = (a.mode == kModeDCT1B) ? R(2) * s : s;
 #pragma unroll
         for (int e = 0; e < E; e++)
-            if (t2 + P * e < a.store_len) dst[obase + (long long)(t2 + P * e) * a.out_sa] = v[e].x * f;
+            if (t2 + P * e < a.store_len) stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * f);
     } else if (a.mode == kModeC2R) {
         R *__restrict__ dst = static_cast<R *>(a.out);
+        if (a.out_sa == 1 && (reinterpret_cast<size_t>(a.out) & (sizeof(C) - 1)) == 0) {
 #pragma unroll
-        for (int e = 0; e < E; e++) {
-            const long long m2 = 2LL * (t2 + P * e);
-            dst[obase + m2 * a.out_sa] = v[e].x * s;
-            dst[obase + (m2 + 1) * a.out_sa] = -v[e].y * s;
+            for (int e = 0; e < E; e++)
+                stg<NT>(reinterpret_cast<C *>(dst + obase) + (t2 + P * e), cmk<C>(v[e].x * s, -v[e].y * s));
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const long long m2 = 2LL * (t2 + P * e);
+                stg<NT>(&dst[obase + m2 * a.out_sa], v[e].x * s);
+                stg<NT>(&dst[obase + (m2 + 1) * a.out_sa], -v[e].y * s);
+            }
         }
     } else if (a.out_real) {
         R *__restrict__ dst = static_cast<R *>(a.out);
 #pragma unroll
-        for (int e = 0; e < E; e++) dst[obase + (long long)(t2 + P * e) * a.out_sa] = v[e].x * s;
+        for (int e = 0; e < E; e++) stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * s);
     } else if (a.store_len < N) {
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
         for (int e = 0; e < E; e++)
             if (t2 + P * e < a.store_len)
-                dst[obase + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+                stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
     } else {
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
         for (int e = 0; e < E; e++)
-            dst[obase + (long long)(t2 + P * e) * a.out_sa] = cmk<C>(v[e].x * s, sg * v[e].y * s);
+            stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
     }
 }
 
-template <typename R, typename RL, int E, int T, int PS>
+template <typename R, typename RL, int E, int T, int PS, bool NT>
 __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     using C = typename cvec<R>::type;
     constexpr int RS = RL::n() + (RL::n() >> PS) + 1;
     __shared__ C lds[T * RS];
-    uniform_tile<R, RL, E, T, PS>(a, blockIdx.x, lds);
+    uniform_tile<R, RL, E, T, PS, NT>(a, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------- generic pass
@@ -530,6 +545,7 @@ __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256) fft_batched(batch_
         a.store_len = x.n[ax];
         a.herm_load = 0;
         a.mode = kModeC2C;
+        a.stream = 0;
         const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
         a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
         batch_tile<R>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);
@@ -594,7 +610,12 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
 
 #define HMI_FAST(TYPE, N, E, T, PS, ...)                                                                  \
     static hipError_t launch_##TYPE##_##N##_##T(const pass_args &a, long long nblocks, hipStream_t s) {    \
-        fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS><<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a); \
+        if (a.stream)                                                                                      \
+            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, true>                                        \
+                <<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a);                                 \
+        else                                                                                               \
+            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, false>                                       \
+                <<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a);                                 \
         return hipGetLastError();                                                                          \
     }
 

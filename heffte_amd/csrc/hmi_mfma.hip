@@ -142,7 +142,7 @@ fft512_mfma(pass_args a) {
         const long long step = (long long)P * a.in_sa;
         float2 x[E];
 #pragma unroll
-        for (int e = 0; e < E; e++) x[e] = p[e * step];
+        for (int e = 0; e < E; e++) x[e] = ld_stream(&p[e * step]);
         float m = 0.f;
 #pragma unroll
         for (int e = 0; e < E; e++) {
@@ -245,7 +245,7 @@ fft512_mfma(pass_args a) {
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int k2 = (e & 3) + 8 * (e >> 2);  // + 4 h, folded into p
-                p[k2 * step] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
+                st_stream(&p[k2 * step], make_float2(acc[e] * s, sg * acc[e + 8] * s));
             }
         }
     }
@@ -304,7 +304,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         const unsigned L = tile * T + l;
         const float2 *p = src + (L < nlines ? line_off(L, a, false) : 0) + (long long)t * a.in_sa;
 #pragma unroll
-        for (int e = 0; e < E; e++) x[e] = p[e * in_step];
+        for (int e = 0; e < E; e++) x[e] = ld_stream(&p[e * in_step]);
     };
     // sign, step-A layout (x[16 n1 + n2] at row n2, column n1; pos = t + 64 e has
     // n2 = t & 15, n1 = (t >> 4) + 4 e), per-line maximum: a full line per wave (along the axis)
@@ -434,7 +434,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                                          (long long)(k1 + 128 * h) * a.out_sa;
                 const long long step = 32 * a.out_sa;
 #pragma unroll
-                for (int e = 0; e < 8; e++) p[((e & 3) + 8 * (e >> 2)) * step] = make_float2(acc[e] * s, sg * acc[e + 8] * s);
+                for (int e = 0; e < 8; e++)
+                    st_stream(&p[((e & 3) + 8 * (e >> 2)) * step], make_float2(acc[e] * s, sg * acc[e + 8] * s));
             }
         }
     }

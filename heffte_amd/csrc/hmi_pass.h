@@ -54,6 +54,7 @@ struct pass_args {
     int store_len; // positions >= store_len along the FFT axis are not stored (r2c half spectrum)
     int herm_load; // 1: positions > n/2 are read as conj(x[n - pos]) (c2r Hermitian extension)
     int mode;      // kModeC2C, kModeR2C or kModeC2R (uniform kernel only)
+    int stream;    // 1: non-temporal loads and stores of the data (outputs larger than the MALL)
     double scale;  // multiplies every stored value (cast to the pass precision)
 };
 

@@ -535,6 +535,11 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         a.store_len = p.store_len;
         a.herm_load = p.herm_load;
         a.mode = p.mode;
+        // the output of a pass that fits the 256 MB Infinity Cache can serve the next pass from
+        // there; larger ones stream past the caches
+        const long long out_bytes = (long long)batch * p.lout.n[0] * p.lout.n[1] * p.lout.n[2] *
+                                    (p.out_real ? (dbl ? 8 : 4) : (dbl ? 16 : 8));
+        a.stream = out_bytes > kStreamBytes ? 1 : 0;
         a.scale = p.last ? scale : 1.0;
         const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
         const bool mfma = precision_ != kPrecPlain && !dbl && p.n == 512 && p.mode == kModeC2C && !p.identity &&

@@ -106,3 +106,31 @@ def test_r2c_large_round_trip(hmi, cuda, dtype):
     tot = p[..., 0].sum() + p[..., -1].sum() + 2 * p[..., 1:-1].sum()
     lhs = (x.double() ** 2).sum() * inbox.count()
     assert abs((tot - lhs) / lhs).item() <= 10 * tol
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_r2c_c2r_unaligned_real_arrays(hmi, cuda, oracle, dtype):
+    """Real arrays starting one element past an aligned address: the contiguous real-data pass
+    must fall back from paired (8/16-byte) accesses to single elements."""
+    import torch
+    rdt, tol = RDT[dtype]
+    size = (64, 12, 10)
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size])
+    outbox = inbox.r2c(0)
+    plan = hmi.fft3d_r2c(hmi.backend.mi355, inbox, outbox, 0)
+    x = np.random.default_rng(11).standard_normal(n)
+    tdt = torch.float32 if dtype == "float32" else torch.float64
+    cdt = torch.complex64 if dtype == "float32" else torch.complex128
+    buf = torch.zeros(n + 1, dtype=tdt, device=cuda)
+    buf[1:] = torch.from_numpy(x).to(tdt)
+    tx = buf[1:]  # odd element offset
+    ty = torch.empty(outbox.count(), dtype=cdt, device=cuda)
+    plan.forward(tx, ty)
+    bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size])
+    ref = oracle.fft3d_r2c(bi, oracle.Box.make((0, 0, 0), [size[0] // 2, size[1] - 1, size[2] - 1]), 0, x)
+    assert fc.rel_l2(fc.to_host(ty), ref) <= tol
+    out = torch.zeros(n + 1, dtype=tdt, device=cuda)
+    plan.backward(ty, out[1:], hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(out[1:]), x) <= tol
+    assert float(out[0]) == 0.0

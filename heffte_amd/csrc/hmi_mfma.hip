@@ -97,6 +97,23 @@ __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bo
                : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
 }
 
+// Offset of line tile * 16 + l.  When the column count is a multiple of 16 the tile's lines
+// share (b1, batch) and the division runs once per tile on uniform values.
+__device__ __forceinline__ long long tile_line_off(unsigned tile, int l, const pass_args &a, bool out) {
+    const long long sc = out ? a.out_sc : a.in_sc;
+    if ((a.ncol & 15) == 0) {
+        const unsigned L0 = tile * 16;
+        const unsigned q = L0 / (unsigned)a.ncol;
+        const unsigned c0 = L0 - q * (unsigned)a.ncol;
+        const unsigned b2 = q / (unsigned)a.nb1;
+        const unsigned b1 = q - b2 * (unsigned)a.nb1;
+        const long long base = out ? (long long)c0 * a.out_sc + (long long)b1 * a.out_sb1 + (long long)b2 * a.out_sb2
+                                   : (long long)c0 * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
+        return base + (long long)l * sc;
+    }
+    return line_off(tile * 16 + l, a, out);
+}
+
 }  // namespace
 
 // One block = 16 lines x 512 points, 1024 threads (16 waves).  a.tw holds the W512^r table
@@ -302,7 +319,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         const int tid = thread();
         const int l = lcf ? (tid & 15) : (tid >> 6), t = lcf ? (tid >> 4) : (tid & 63);
         const unsigned L = tile * T + l;
-        const float2 *p = src + (L < nlines ? line_off(L, a, false) : 0) + (long long)t * a.in_sa;
+        const float2 *p = src + (L < nlines ? tile_line_off(tile, l, a, false) : 0) + (long long)t * a.in_sa;
 #pragma unroll
         for (int e = 0; e < E; e++) x[e] = ld_stream(&p[e * in_step]);
     };
@@ -341,18 +358,11 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         return m;
     };
 
-    unsigned tile = blockIdx.x;
-    {
-        float2 x[E];
-        issue(tile, x);
-        deposit(tile, x, 0);
-    }
-    __syncthreads();
-    for (int it = 0; tile < ntiles; it++, tile += gridDim.x) {
-        const int par = it & 1;
+    // one tile t: loads of tile t + G into xn | step A | barrier | Y -> LDS, tile t + G -> LDS |
+    // barrier | step C, store
+    auto run_tile = [&](unsigned tile, int par, float2 (&xn)[E]) {
         const unsigned next = tile + gridDim.x;
         const bool has_next = next < ntiles;
-        float2 xn[E];
         if (has_next) issue(next, xn);
 
         // ---- step A: wave -> (column block b, real / imaginary rows m); column j = (n2, line)
@@ -430,7 +440,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                 int ex = 0;
                 (void)frexpf(line_max(par, line), &ex);
                 const float s = ldexpf((float)a.scale, ex);
-                float2 *__restrict__ p = static_cast<float2 *>(a.out) + line_off(L, a, true) +
+                float2 *__restrict__ p = static_cast<float2 *>(a.out) + tile_line_off(tile, line, a, true) +
                                          (long long)(k1 + 128 * h) * a.out_sa;
                 const long long step = 32 * a.out_sa;
 #pragma unroll
@@ -438,6 +448,18 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                     st_stream(&p[((e & 3) + 8 * (e >> 2)) * step], make_float2(acc[e] * s, sg * acc[e + 8] * s));
             }
         }
+    };
+
+    unsigned tile = blockIdx.x;
+    {
+        float2 x[E];
+        issue(tile, x);
+        deposit(tile, x, 0);
+    }
+    __syncthreads();
+    for (int it = 0; tile < ntiles; it++, tile += gridDim.x) {
+        float2 xn[E];
+        run_tile(tile, it & 1, xn);
     }
 }
 

@@ -116,191 +116,57 @@ __device__ __forceinline__ long long tile_line_off(unsigned tile, int l, const p
 
 }  // namespace
 
-// One block = 16 lines x 512 points, 1024 threads (16 waves).  a.tw holds the W512^r table
+// One tile = 16 lines x 512 points for 1024 threads (16 waves).  a.tw holds the W512^r table
 // (512 complex<float>) followed by the A operands of both GEMMs, already split and laid out
 // per lane (mfma512_operand_table): step A [m][ks][term][lane][8], step C [ks][term][lane][8].
+// LDS holds the tile in the layout each GEMM reads with contiguous K: x[16 n1 + n2] at
+// (line, n2, n1) for step A, Y[k1][n2] at (line, k1, n2) for step C (odd row pitches spread
+// the rows over the banks).
 //
-// LDS holds the tile twice over its life, in the layout each GEMM reads with contiguous
-// K: x[16 n1 + n2] at LA(line, n2, n1) for step A, Y[k1][n2] at LC(line, k1, n2) for step C
-// (odd row pitches spread the rows over the banks).
-template <int PREC>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8)))
-fft512_mfma(pass_args a) {
-    constexpr int T = 16, P = 64, E = 8;
-    constexpr int RA = 33, RLA = 16 * RA + 1;  // step A rows: n2 x 33
-    constexpr int RC = 17, RLC = 32 * RC + 1;  // step C rows: k1 x 17
-    constexpr int LDS = (T * RLA > T * RLC) ? T * RLA : T * RLC;
-    using S = split_traits<PREC>;
-    using vec = typename S::vec;
-    constexpr int TERMS = S::terms;
-    __shared__ float2 lds[LDS];
-    __shared__ unsigned lmax[T];
-
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const unsigned tile = blockIdx.x;
-    const float sg = a.conj ? -1.0f : 1.0f;
-    const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
-    const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);  // [2][4][TERMS][64]
-    const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;                    // [2][TERMS][64]
-    float *ldsf = reinterpret_cast<float *>(lds);
-
-    if (tid < T) lmax[tid] = 0u;
-    __syncthreads();
-    // ---- load the tile into the step-A layout; per-line maximum of |re|, |im|
-    {
-        int l, t;
-        if (a.load_colfast) { l = tid % T; t = tid / T; } else { l = tid / P; t = tid % P; }
-        const unsigned L = tile * T + l;
-        const bool valid = L < (unsigned)a.nlines;
-        const long long base = valid ? line_off(L, a, false) : 0;
-        // all eight loads in flight before the first use (an invalid line reads line 0)
-        const float2 *__restrict__ p = static_cast<const float2 *>(a.in) + base + (long long)t * a.in_sa;
-        const long long step = (long long)P * a.in_sa;
-        float2 x[E];
-#pragma unroll
-        for (int e = 0; e < E; e++) x[e] = ld_stream(&p[e * step]);
-        float m = 0.f;
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            const int pos = t + P * e;
-            if (!valid) x[e] = make_float2(0.f, 0.f);
-            x[e].y *= sg;
-            m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
-            lds[l * RLA + (pos & 15) * RA + (pos >> 4)] = x[e];
-        }
-        // lanes holding the same line: all 64 (along the axis) or lanes l, l+16, l+32, l+48
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        if (!a.load_colfast) {
-#pragma unroll
-            for (int off = 8; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        }
-        if ((a.load_colfast && lane < 16) || (!a.load_colfast && lane == 0)) atomicMax(&lmax[l], __float_as_uint(m));
-    }
-    __syncthreads();
-
-    const int j = lane & 31, h = lane >> 5;
-    // ---- step A: wave -> (column block b, real/imaginary output m); column j = (n2, line)
-    f32x16_t acc = {};
-    {
-        const int b = wave >> 1, m = wave & 1;
-        const int n2 = j & 15, line = 2 * b + (j >> 4);
-        int ex = 0;
-        (void)frexpf(__uint_as_float(lmax[line]), &ex);  // line max = f 2^ex, 0.5 <= f < 1
-        const float s = ldexpf(1.0f, -ex);
-#pragma unroll
-        for (int half = 0; half < 2; half++) {
-            float re[8], im[8];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                // n1 = 16 half + 8 h + 2 q, +1: two complex values per 16-byte read
-                const float2 *p2 = &lds[line * RLA + n2 * RA + 16 * half + 8 * h + 2 * q];
-                const float2 x0 = p2[0], x1 = p2[1];
-                re[2 * q] = x0.x * s;
-                im[2 * q] = x0.y * s;
-                re[2 * q + 1] = x1.x * s;
-                im[2 * q + 1] = x1.y * s;
-            }
-            // K-step `half` multiplies the real parts, K-step `2 + half` the imaginary parts
-#pragma unroll
-            for (int part = 0; part < 2; part++) {
-                const int ks = half + 2 * part;
-                vec A[3], B[3];
-#pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) A[tt] = opA[((m * 4 + ks) * TERMS + tt) * 64 + lane];
-                split8<PREC>(part == 0 ? re : im, B);
-                acc = split_mma<PREC>(A, B, acc);
-            }
-        }
-    }
-    __syncthreads();  // every wave has read its operands: the tile may be overwritten
-    {
-        const int b = wave >> 1, m = wave & 1;
-        const int n2 = j & 15, line = 2 * b + (j >> 4);
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int k1 = (e & 3) + 8 * (e >> 2) + 4 * h;
-            ldsf[2 * (line * RLC + k1 * RC + n2) + m] = acc[e];
-        }
-    }
-    __syncthreads();
-    // ---- twiddle + step C: wave -> 32 columns (k1, line)
-    int line, k1;
-    if (a.store_colfast) { line = j & 15; k1 = 2 * wave + (j >> 4); }
-    else { line = wave; k1 = j; }
-    acc = f32x16_t{};
-    {
-        float zr[8], zi[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int n2 = 8 * h + e;
-            const float2 y = lds[line * RLC + k1 * RC + n2];
-            const float2 w = w512[n2 * k1];  // n2 * k1 <= 465
-            zr[e] = y.x * w.x - y.y * w.y;
-            zi[e] = y.x * w.y + y.y * w.x;
-        }
-#pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            vec A[3], B[3];
-#pragma unroll
-            for (int tt = 0; tt < TERMS; tt++) A[tt] = opC[(ks * TERMS + tt) * 64 + lane];
-            split8<PREC>(ks == 0 ? zr : zi, B);
-            acc = split_mma<PREC>(A, B, acc);
-        }
-    }
-    // ---- store: lane holds X[k1 + 32 k2] for k2 = (e & 3) + 8 (e >> 2) + 4 h, e < 8
-    {
-        const unsigned L = tile * T + line;
-        if (L < (unsigned)a.nlines) {
-            int ex = 0;
-            (void)frexpf(__uint_as_float(lmax[line]), &ex);
-            const float s = ldexpf((float)a.scale, ex);  // undo the line scale
-            float2 *__restrict__ p = static_cast<float2 *>(a.out) + line_off(L, a, true) +
-                                     (long long)(k1 + 128 * h) * a.out_sa;
-            const long long step = 32 * a.out_sa;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int k2 = (e & 3) + 8 * (e >> 2);  // + 4 h, folded into p
-                st_stream(&p[k2 * step], make_float2(acc[e] * s, sg * acc[e + 8] * s));
-            }
-        }
-    }
-}
-
 // Persistent form: one 1024-thread block per CU walks the tiles blockIdx.x, blockIdx.x +
-// gridDim.x, ...  The split DFT operands of both GEMMs and W512 are copied to LDS once, and
+// gridDim.x, ...  The split DFT operands (and, for FP16, W512) are copied to LDS once, and
 // the global loads of the next tile are in flight while the current tile runs step A.  Both
 // tile layouts, the tables and the per-line maxima (by tile parity: one partial per wave and
-// line) take 163,728 B of LDS: one block per CU.
+// line) take 163,728 B (FP16) or 163,840 B (BF16) of LDS: one block per CU.
 // Per tile: issue loads(next) | step A | barrier | Y -> LDS, next -> LDS | barrier | step C, store.
 template <int PREC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     constexpr int T = 16, P = 64, E = 8;
-    constexpr int RA = 33, RLA = 16 * RA + 1;
-    constexpr int RC = 17, RLC = 32 * RC + 1;
     using S = split_traits<PREC>;
     using vec = typename S::vec;
     constexpr int TERMS = S::terms;
-    __shared__ float2 ldsA[T * RLA];
-    __shared__ float2 ldsC[T * RLC];
-    __shared__ float2 w512s[466];  // W512^r for r = n2 k1 <= 15 * 31
-    __shared__ vec opAs[2 * 4 * TERMS * 64];
-    __shared__ vec opCs[2 * TERMS * 64];
-    __shared__ float pmax[2][T][T];
+    // The three-term BF16 operands leave no LDS for W512 and the step-C operands (read from
+    // the cached table instead) nor for the one-element pad between lines.
+    constexpr bool kTablesInLds = (TERMS == 2);
+    constexpr int PAD = kTablesInLds ? 1 : 0;
+    constexpr int RA = 33, RLA = 16 * RA + PAD;
+    constexpr int RC = 17, RLC = 32 * RC + PAD;
+    constexpr size_t kA = 0, kC = kA + sizeof(float2) * T * RLA, kOpA = kC + sizeof(float2) * T * RLC;
+    constexpr size_t kMax = kOpA + sizeof(vec) * 2 * 4 * TERMS * 64, kW = kMax + sizeof(float) * 2 * T * T;
+    constexpr size_t kOpC = kW + (kTablesInLds ? sizeof(float2) * 466 : 0);
+    constexpr size_t kBytes = kOpC + (kTablesInLds ? sizeof(vec) * 2 * TERMS * 64 : 0);
+    __shared__ __align__(16) unsigned char lds[kBytes];
+    float2 *ldsA = reinterpret_cast<float2 *>(lds + kA);
+    float2 *ldsC = reinterpret_cast<float2 *>(lds + kC);
+    vec *opAs = reinterpret_cast<vec *>(lds + kOpA);
+    float(*pmax)[T][T] = reinterpret_cast<float(*)[T][T]>(lds + kMax);
+    float2 *w512s = reinterpret_cast<float2 *>(lds + kW);  // W512^r for r = n2 k1 <= 15 * 31
+    vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
 
     const float sg = a.conj ? -1.0f : 1.0f;
     const unsigned nlines = (unsigned)a.nlines;
     const bool lcf = a.load_colfast != 0, scf = a.store_colfast != 0;
+    const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
+    const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);
+    const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;
     {
-        const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
-        const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);
-        const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;
         const int tid = threadIdx.x;
         for (int i = tid; i < 2 * 4 * TERMS * 64; i += 1024) opAs[i] = opA[i];
-        if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
-        if (tid < 466) w512s[tid] = w512[tid];
+        if constexpr (kTablesInLds) {
+            if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
+            if (tid < 466) w512s[tid] = w512[tid];
+        }
     }
     float *ldsCf = reinterpret_cast<float *>(ldsC);
     const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
@@ -418,11 +284,11 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
             acc = f32x16_t{};
             float zr[8], zi[8];
             const float2 *yp = &ldsC[line * RLC + k1 * RC + 8 * h];
-            const float2 *wp = &w512s[8 * h * k1];
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const float2 y = yp[e];
-                const float2 w = wp[e * k1];  // W512^{n2 k1}, n2 = 8 h + e
+                // W512^{n2 k1}, n2 = 8 h + e
+                const float2 w = kTablesInLds ? w512s[(8 * h + e) * k1] : w512[(8 * h + e) * k1];
                 zr[e] = y.x * w.x - y.y * w.y;
                 zi[e] = y.x * w.y + y.y * w.x;
             }
@@ -430,7 +296,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
             for (int ks = 0; ks < 2; ks++) {
                 vec A[3], B[3];
 #pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) A[tt] = opCs[(ks * TERMS + tt) * 64 + lane];
+                for (int tt = 0; tt < TERMS; tt++)
+                    A[tt] = kTablesInLds ? opCs[(ks * TERMS + tt) * 64 + lane] : opC[(ks * TERMS + tt) * 64 + lane];
                 split8<PREC>(ks == 0 ? zr : zi, B);
                 acc = split_mma<PREC>(A, B, acc);
             }
@@ -537,7 +404,12 @@ hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipSt
         const long long grid = std::min<long long>(nblocks, std::max(1, cus));
         fft512_mfma_persistent<kSplitF16><<<dim3((unsigned)grid), dim3(1024), 0, s>>>(a, (unsigned)nblocks);
     } else if (prec == kSplitBF16) {
-        fft512_mfma<kSplitBF16><<<dim3((unsigned)nblocks), dim3(1024), 0, s>>>(a);
+        int dev = 0, cus = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) return e;
+        const long long grid = std::min<long long>(nblocks, std::max(1, cus));
+        fft512_mfma_persistent<kSplitBF16><<<dim3((unsigned)grid), dim3(1024), 0, s>>>(a, (unsigned)nblocks);
     } else {
         return hipErrorInvalidValue;
     }

@@ -77,3 +77,40 @@ def test_split_form_ignored_for_double_and_other_lengths(hmi, cuda, oracle):
     plan.forward(tx, ty)
     b = oracle.Box.make((0, 0, 0), [s - 1 for s in size])
     assert fc.rel_l2(fc.to_host(ty), oracle.fft3d_c2c(b, b, x, -1, 1.0)) <= 1e-13
+
+
+def power_law_field(size, seed, slope=-5.0 / 3.0):
+    """Complex Gaussian random field with power spectrum |k|^slope (x fastest, flattened)."""
+    rng = np.random.default_rng(seed)
+    n0, n1, n2 = size
+    k0 = np.fft.fftfreq(n0) * n0
+    k1 = np.fft.fftfreq(n1) * n1
+    k2 = np.fft.fftfreq(n2) * n2
+    kk = np.sqrt(k2[:, None, None] ** 2 + k1[None, :, None] ** 2 + k0[None, None, :] ** 2)
+    kk[0, 0, 0] = 1.0
+    amp = kk ** (slope / 2.0)
+    white = rng.standard_normal((n2, n1, n0)) + 1j * rng.standard_normal((n2, n1, n0))
+    return np.fft.ifftn(white * amp).reshape(-1)
+
+
+@pytest.mark.parametrize("form", ["fp32", "split_fp16", "split_bf16"])
+def test_power_law_field(hmi, cuda, oracle, form):
+    """Gaussian random field with a |k|^-5/3 spectrum (seed 3): relative L2 within 5e-6 and the
+    maximum error relative to the largest output magnitude stated."""
+    import torch
+    size = (512, 64, 16)
+    x = power_law_field(size, 3)
+    box = hmi.box3d([0, 0, 0], [s - 1 for s in size])
+    plan = hmi.fft3d(hmi.backend.mi355, box, box)
+    plan.set_precision(getattr(hmi.precision, form))
+    tx = fc.to_gpu(x, np.complex64)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    b = oracle.Box.make((0, 0, 0), [s - 1 for s in size])
+    ref = oracle.fft3d_c2c(b, b, x.astype(np.complex64).astype(np.complex128), -1, 1.0)
+    y = fc.to_host(ty)
+    err = fc.rel_l2(y, ref)
+    mx = np.abs(y - ref).max() / np.abs(ref).max()
+    print(form, f"|k|^-5/3 field forward rel L2 {err:.2e}, max {mx:.2e}")
+    assert err <= 5e-6
+    assert mx <= 5e-6

@@ -66,6 +66,7 @@ def _load():
     lib.heffte_mi355_batch_destroy.argtypes = [plan_p]
     lib.heffte_mi355_batch_set_stream.argtypes = [plan_p, vp]
     lib.heffte_mi355_batch_set_graph.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_batch_set_autotune.argtypes = [plan_p, ctypes.c_int]
     for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output"):
         getattr(lib, f).argtypes = [plan_p]
         getattr(lib, f).restype = ctypes.c_longlong
@@ -92,7 +93,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_create", "heffte_mi355_batch_destroy", "heffte_mi355_batch_set_stream",
     "heffte_mi355_batch_set_graph", "heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output",
     "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
-    "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision",
+    "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
 )
 
 
@@ -390,6 +391,10 @@ class batch_plan:
 
     def set_graph(self, enable):
         _check(lib.heffte_mi355_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
+
+    def set_autotune(self, enable):
+        """Time the tile candidates of each side length on the first call (default) or not."""
+        _check(lib.heffte_mi355_batch_set_autotune(self._h, 1 if enable else 0), "set_autotune")
 
     def _run(self, direction, inarray, outarray, scaling):
         t = _dtype_name(inarray)

@@ -1,7 +1,15 @@
 // Batched mixed-size c2c transforms (see hmi_batch.h).
 #include "hmi_batch.h"
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <limits>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -83,8 +91,11 @@ void batch_plan::prepare(bool dbl) {
     std::vector<double> td;
     size_t lds = 0;
     for (int n : lengths_) {
+        auto fc = forced_.find(n);
+        const batch_choice c = fc != forced_.end() ? fc->second
+                                                   : (autotune_ ? batch_tuner::choose(n, dbl) : batch_tuner::fallback(n, dbl));
         batch_kernel_info info{};
-        const int kid = find_batch_kernel(n, &info);
+        const int kid = c.table ? find_batch_kernel(n, &info) : -1;
         generic_plan g{};
         if (kid >= 0) {
             // compile-time tile: only its line count is read from the plan
@@ -93,10 +104,8 @@ void batch_plan::prepare(bool dbl) {
             lds = std::max(lds, dbl ? info.lds_f64 : info.lds_f32);
         } else {
             const size_t per_line = 2 * (size_t)n * elem;
-            // keep every tile of the batch within 40 KiB of LDS: several resident blocks per CU
-            const int lines = (int)std::min<size_t>(64, std::max<size_t>(1, (40 * 1024) / per_line));
-            g = make_generic_plan(n, lines);
-            lds = std::max(lds, per_line * lines);
+            g = make_generic_plan(n, std::max(1, c.lines), c.radices.empty() ? default_radices(n) : c.radices);
+            lds = std::max(lds, per_line * g.lines);
         }
         if (dbl) {
             twoff.push_back((int)(td.size() / 2));
@@ -194,6 +203,189 @@ void batch_plan::execute(int dir, bool dbl, const void *in, void *out, int scali
         gkey_[g] = key;
     }
     check_hip(hipGraphLaunch(graph_[g], stream_), "hipGraphLaunch");
+}
+
+
+// ------------------------------------------------------------------ tuner
+
+namespace {
+
+std::mutex g_tune_mtx;
+std::map<std::string, std::map<long long, batch_choice>> g_tuned;  // cache path -> (n, dbl) -> choice
+
+long long tune_key(int n, bool dbl) { return 2LL * n + (dbl ? 1 : 0); }
+
+std::string cache_path() {
+    if (const char *p = std::getenv("HMI_TUNE_CACHE")) return p;
+    if (const char *h = std::getenv("HOME")) return std::string(h) + "/.cache/heffte_mi355/batch_tuning.txt";
+    return "";
+}
+
+std::map<long long, batch_choice> load_cache(const std::string &path) {
+    std::map<long long, batch_choice> m;
+    if (path.empty()) return m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream in(line);
+        int n = 0, dbl = 0, table = 0, lines = 0;
+        std::string rads;
+        if (!(in >> n >> dbl >> table >> lines >> rads) || n < 1) continue;
+        batch_choice c;
+        c.table = table != 0;
+        c.lines = lines;
+        long long prod = 1;
+        std::istringstream rs(rads);
+        std::string tok;
+        while (std::getline(rs, tok, ','))
+            if (!tok.empty() && tok != "-") {
+                c.radices.push_back(std::atoi(tok.c_str()));
+                prod *= c.radices.back();
+            }
+        // entries that no longer describe a valid plan are ignored (and tuned again)
+        if (c.table ? find_batch_kernel(n, nullptr) < 0 : (lines < 1 || prod != n)) continue;
+        m[tune_key(n, dbl != 0)] = c;
+    }
+    return m;
+}
+
+void save_cache(const std::string &path, const std::map<long long, batch_choice> &m) {
+    if (path.empty()) return;
+    const size_t slash = path.find_last_of('/');
+    if (slash != std::string::npos) {
+        // create the directory chain; failures show up as a failed open below
+        for (size_t i = 1; i <= slash; i++)
+            if (path[i] == '/') (void)mkdir(path.substr(0, i).c_str(), 0755);
+    }
+    const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+    {
+        std::ofstream f(tmp);
+        if (!f) return;  // a read-only location only loses the cache
+        for (const auto &kv : m) {
+            const batch_choice &c = kv.second;
+            f << (kv.first / 2) << ' ' << (kv.first & 1) << ' ' << (c.table ? 1 : 0) << ' ' << c.lines << ' ';
+            if (c.radices.empty()) f << '-';
+            for (size_t i = 0; i < c.radices.size(); i++) f << (i ? "," : "") << c.radices[i];
+            f << '\n';
+        }
+    }
+    (void)std::rename(tmp.c_str(), path.c_str());
+}
+
+}  // namespace
+
+// Fastest of five runs of a probe batch (one n x n x m transform, m a power of two with
+// about 2^20 elements in all), forward, in place, on a stream of its own.
+double batch_tuner::time_choice(int n, bool dbl, const batch_choice &c) {
+    long long m = 16;
+    while (m < 512 && (long long)n * n * m * 2 <= (1LL << 20)) m *= 2;
+    const int sizes[3] = {n, n, (int)m};
+    batch_plan probe(1, sizes, nullptr, nullptr);
+    probe.set_graph(false);
+    probe.set_autotune(false);
+    probe.forced_[n] = c;
+    const size_t bytes = (size_t)n * n * m * (dbl ? 16 : 8);
+    void *buf = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double best = std::numeric_limits<double>::infinity();
+    try {
+        check_hip(hipMalloc(&buf, bytes), "hipMalloc(tuning probe)");
+        check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate(tuning)");
+        check_hip(hipMemsetAsync(buf, 0, bytes, s), "hipMemsetAsync(tuning probe)");
+        check_hip(hipEventCreate(&e0), "hipEventCreate");
+        check_hip(hipEventCreate(&e1), "hipEventCreate");
+        probe.set_stream(s);
+        probe.execute(-1, dbl, buf, buf, 0);
+        for (int r = 0; r < 5; r++) {
+            check_hip(hipEventRecord(e0, s), "hipEventRecord");
+            probe.execute(-1, dbl, buf, buf, 0);
+            check_hip(hipEventRecord(e1, s), "hipEventRecord");
+            check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            best = std::min(best, (double)ms);
+        }
+    } catch (...) {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamSynchronize(s), (void)hipStreamDestroy(s);
+        if (buf) (void)hipFree(buf);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(buf);
+    return best;
+}
+
+batch_choice batch_tuner::fallback(int n, bool dbl) {
+    batch_choice c;
+    if (find_batch_kernel(n, nullptr) >= 0) {
+        c.table = true;
+        return c;
+    }
+    const size_t per_line = 2 * (size_t)n * (dbl ? 16 : 8);
+    // every tile of the batch within 40 KiB of LDS: several resident blocks per CU
+    c.lines = (int)std::min<size_t>(64, std::max<size_t>(1, (40 * 1024) / per_line));
+    c.radices = default_radices(n);
+    return c;
+}
+
+std::vector<batch_choice> batch_tuner::candidates(int n, bool dbl) {
+    std::vector<batch_choice> v;
+    if (find_batch_kernel(n, nullptr) >= 0) {
+        batch_choice t;
+        t.table = true;
+        v.push_back(t);
+    }
+    const size_t per_line = 2 * (size_t)n * (dbl ? 16 : 8);
+    const int lmax = (int)std::min<size_t>(64, std::max<size_t>(1, (40 * 1024) / per_line));
+    const std::vector<int> rad = default_radices(n);
+    for (int lines : {lmax, lmax / 2, lmax / 4}) {
+        if (lines < 1) continue;
+        bool seen = false;
+        for (const batch_choice &c : v) seen = seen || (!c.table && c.lines == lines);
+        if (seen) continue;
+        batch_choice c;
+        c.lines = lines;
+        c.radices = rad;
+        v.push_back(c);
+    }
+    std::vector<int> rev(rad.rbegin(), rad.rend());
+    if (rev != rad) {
+        batch_choice c;
+        c.lines = lmax;
+        c.radices = rev;
+        v.push_back(c);
+    }
+    return v;
+}
+
+batch_choice batch_tuner::choose(int n, bool dbl) {
+    const std::string path = cache_path();
+    std::lock_guard<std::mutex> lock(g_tune_mtx);
+    auto pit = g_tuned.find(path);
+    if (pit == g_tuned.end()) pit = g_tuned.emplace(path, load_cache(path)).first;
+    auto it = pit->second.find(tune_key(n, dbl));
+    if (it != pit->second.end()) return it->second;
+    const std::vector<batch_choice> cands = candidates(n, dbl);
+    batch_choice best = fallback(n, dbl);
+    if (cands.size() > 1) {
+        double tbest = std::numeric_limits<double>::infinity();
+        for (const batch_choice &c : cands) {
+            const double t = time_choice(n, dbl, c);
+            if (t < tbest) {
+                tbest = t;
+                best = c;
+            }
+        }
+    }
+    pit->second[tune_key(n, dbl)] = best;
+    save_cache(path, pit->second);
+    return best;
 }
 
 }  // namespace hmi

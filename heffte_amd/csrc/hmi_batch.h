@@ -7,11 +7,35 @@
 
 #include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "hmi_kernels.h"
 
 namespace hmi {
+
+// How the batched kernel runs one axis length: the compile-time tile of that length, or the
+// runtime-plan tile with `lines` lines per tile and the given radix order.
+struct batch_choice {
+    bool table = false;
+    int lines = 0;
+    std::vector<int> radices;
+};
+
+// Picks the batch_choice of each (length, precision) by timing the candidates on the current
+// GPU (a probe batch of n x n x m transforms, fastest of five runs) and keeps the result in
+// memory and in a text cache file: $HMI_TUNE_CACHE, else $HOME/.cache/heffte_mi355/
+// batch_tuning.txt (one line per choice: "n dbl table lines r1,r2,...").
+class batch_tuner {
+  public:
+    static batch_choice choose(int n, bool dbl);
+    // the untimed default: the compile-time tile if there is one, else the runtime plan with
+    // as many lines as fit 40 KiB of LDS
+    static batch_choice fallback(int n, bool dbl);
+    static std::vector<batch_choice> candidates(int n, bool dbl);
+    // milliseconds of the probe batch run with choice c for length n
+    static double time_choice(int n, bool dbl, const batch_choice &c);
+};
 
 class batch_plan {
   public:
@@ -21,6 +45,8 @@ class batch_plan {
 
     void set_stream(hipStream_t s) { stream_ = s; }
     void set_graph(bool on) { use_graph_ = on; }
+    // off: every length takes batch_tuner::fallback (no timing runs on the first execute)
+    void set_autotune(bool on) { autotune_ = on; }
     int count() const { return (int)xf_.size(); }
     long long input_elems() const { return in_extent_; }
     long long output_elems() const { return out_extent_; }
@@ -56,6 +82,9 @@ class batch_plan {
     long long in_extent_ = 0, out_extent_ = 0;
     hipStream_t stream_ = nullptr;
     bool use_graph_ = true;
+    bool autotune_ = true;
+    std::map<int, batch_choice> forced_;  // probe plans of the tuner
+    friend class batch_tuner;
     device_state dev_[2];
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
     graph_key gkey_[2] = {};

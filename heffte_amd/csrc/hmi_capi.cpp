@@ -244,6 +244,12 @@ int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable) {
     return HMI_SUCCESS;
 }
 
+int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    batch->impl->set_autotune(enable != 0);
+    return HMI_SUCCESS;
+}
+
 long long heffte_mi355_batch_size_input(heffte_mi355_batch batch) {
     return valid_batch(batch) ? batch->impl->input_elems() : -1;
 }

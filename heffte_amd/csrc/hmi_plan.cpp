@@ -405,10 +405,16 @@ std::vector<R> make_table(const int *radix, int nstages, bool with_roots, int ta
 
 }  // namespace
 
-generic_plan make_generic_plan(int n, int lines) {
+std::vector<int> default_radices(int n) { return factorize(n); }
+
+generic_plan make_generic_plan(int n, int lines) { return make_generic_plan(n, lines, factorize(n)); }
+
+generic_plan make_generic_plan(int n, int lines, const std::vector<int> &rad) {
     generic_plan g{};
     g.n = n;
-    std::vector<int> rad = factorize(n);
+    long long prod = 1;
+    for (int r : rad) prod *= r;
+    if (prod != n || (rad.empty() && n != 1)) throw std::invalid_argument("radices do not multiply to the length");
     if ((int)rad.size() > kMaxStages) throw std::runtime_error("too many radix stages");
     g.nstages = (int)rad.size();
     int off = 0, ls = 1;

@@ -121,6 +121,10 @@ class fft_plan {
 // Runtime radix plan of length n for the generic tile (stage/root/post offsets
 // filled in; lines = lines per tile) and its twiddle table in either precision.
 generic_plan make_generic_plan(int n, int lines);
+// The same with an explicit radix order (their product must be n).
+generic_plan make_generic_plan(int n, int lines, const std::vector<int> &radices);
+// The radix order the planner uses by default: 8s, then a 4 and a 2, then odd primes.
+std::vector<int> default_radices(int n);
 std::vector<float> generic_table_f32(const generic_plan &g);
 std::vector<double> generic_table_f64(const generic_plan &g);
 // Stage twiddles of a compile-time radix plan (no root or real-data tables).

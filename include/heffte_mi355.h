@@ -170,6 +170,13 @@ int heffte_mi355_batch_create(int count, int const *sizes, long long const *in_o
 int heffte_mi355_batch_destroy(heffte_mi355_batch batch);
 int heffte_mi355_batch_set_stream(heffte_mi355_batch batch, void *hip_stream);
 int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
+/* Autotuning (on by default): on the first forward/backward call of each precision, every
+ * distinct side length of the batch gets its radix plan, lines per tile and LDS staging (the
+ * compile-time tile or a runtime plan) by timing the candidates on the GPU; choices are cached
+ * per (length, precision) in memory and in $HMI_TUNE_CACHE (default
+ * $HOME/.cache/heffte_mi355/batch_tuning.txt).  enable = 0 takes the untimed defaults.  The
+ * first call then runs timing launches on a stream of its own: do it outside stream capture. */
+int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable);
 /* Elements (complex) the input / output arrays must hold. */
 long long heffte_mi355_batch_size_input(heffte_mi355_batch batch);
 long long heffte_mi355_batch_size_output(heffte_mi355_batch batch);

@@ -121,3 +121,37 @@ def test_batch_matches_single_plans(hmi, cuda):
         p.forward(tx[off:off + n], out)
         assert fc.rel_l2(fc.to_host(ty[off:off + n]), fc.to_host(out).astype(np.complex128)) <= 5e-6
         off += n
+
+
+def test_autotune_cache_and_parity(hmi, cuda, oracle, tmp_path, monkeypatch):
+    """The tuner times the candidate tiles of every side length on the first call, writes one
+    cache line per (length, precision), and the tuned plan matches the oracle; a second plan
+    reads the cache and gives the same result bit for bit; untimed defaults also pass."""
+    import torch
+    cache = tmp_path / "sub" / "tuning.txt"
+    monkeypatch.setenv("HMI_TUNE_CACHE", str(cache))
+    sizes = [(20, 48, 105), (48, 20, 32), (105, 105, 16)]
+    counts = [int(np.prod(s)) for s in sizes]
+    offsets = np.cumsum([0] + counts)[:-1]
+    plan = hmi.batch_plan(sizes)
+    x = fc.uniform_complex(plan.size_input(), 21)
+    tx = torch.from_numpy(x).to(cuda).to(torch.complex64)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    y = fc.to_host(ty)
+    for s, off, r in zip(sizes, offsets, oracle_each(oracle, sizes, offsets, x, -1, 0)):
+        assert fc.rel_l2(y[off:off + r.size], r) <= 5e-6, s
+    rows = [line.split() for line in cache.read_text().splitlines() if line.strip()]
+    tuned = {(int(r[0]), int(r[1])) for r in rows}
+    assert {(20, 0), (48, 0), (105, 0), (32, 0), (16, 0)} <= tuned
+    for r in rows:
+        assert r[2] in ("0", "1") and int(r[3]) >= (0 if r[2] == "1" else 1)
+    plan2 = hmi.batch_plan(sizes)
+    ty2 = torch.empty_like(tx)
+    plan2.forward(tx, ty2)
+    assert torch.equal(ty, ty2)
+    plan3 = hmi.batch_plan(sizes)
+    plan3.set_autotune(False)
+    ty3 = torch.empty_like(tx)
+    plan3.forward(tx, ty3)
+    assert fc.rel_l2(fc.to_host(ty3), y.astype(np.complex128)) <= 5e-6

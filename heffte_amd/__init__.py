@@ -19,7 +19,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY_PATH = os.path.join(_HERE, "libheffte_mi355.so")
+# HEFFTE_MI355_LIBRARY selects another build of the same library (e.g. to compare two builds)
+LIBRARY_PATH = os.environ.get("HEFFTE_MI355_LIBRARY") or os.path.join(_HERE, "libheffte_mi355.so")
 
 
 class heffte_input_error(Exception):

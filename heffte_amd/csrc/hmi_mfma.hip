@@ -154,7 +154,6 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     float2 *w512s = reinterpret_cast<float2 *>(lds + kW);  // W512^r for r = n2 k1 <= 15 * 31
     vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
 
-    const float sg = a.conj ? -1.0f : 1.0f;
     const unsigned nlines = (unsigned)a.nlines;
     const bool lcf = a.load_colfast != 0, scf = a.store_colfast != 0;
     const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
@@ -202,7 +201,6 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 #pragma unroll
         for (int e = 0; e < E; e++) {
             if (!valid) x[e] = make_float2(0.f, 0.f);
-            x[e].y *= sg;
             m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
             row[4 * e] = x[e];
         }
@@ -312,7 +310,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                 const long long step = 32 * a.out_sa;
 #pragma unroll
                 for (int e = 0; e < 8; e++)
-                    st_stream(&p[((e & 3) + 8 * (e >> 2)) * step], make_float2(acc[e] * s, sg * acc[e + 8] * s));
+                    st_stream(&p[((e & 3) + 8 * (e >> 2)) * step], make_float2(acc[e] * s, acc[e + 8] * s));
             }
         }
     };
@@ -333,10 +331,11 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 namespace {
 long double cpi() { return 3.141592653589793238462643383279502884L; }
 // exp(-2 pi i r / m) in long double
-void wroot(long long r, long long m, long double &re, long double &im) {
+// exp(-2 pi i r / m) in long double, or its conjugate (conj: the backward transform's roots)
+void wroot(long long r, long long m, long double &re, long double &im, bool conj) {
     const long double ang = 2.0L * cpi() * (long double)(r % m) / (long double)m;
     re = cosl(ang);
-    im = -sinl(ang);
+    im = conj ? sinl(ang) : -sinl(ang);
 }
 template <typename S> void split_store(float v, int terms, S *out, int stride) {
     float r = v;
@@ -346,7 +345,7 @@ template <typename S> void split_store(float v, int terms, S *out, int stride) {
         r -= (float)p;
     }
 }
-template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int terms) {
+template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int terms, bool conj) {
     // step A: A[i][k] of [Re F32, -Im F32; Im F32, Re F32], rows i = k1 (m = 0: real rows,
     // m = 1: imaginary rows), K-step ks covers k = 16 ks + 8 (lane / 32) + e
     const size_t nA = 2 * 4 * terms * 64 * 8, nC = 2 * terms * 64 * 8;
@@ -357,7 +356,7 @@ template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int 
                 for (int e = 0; e < 8; e++) {
                     const int i = lane & 31, n1 = 16 * (ks & 1) + 8 * (lane >> 5) + e;
                     long double wr, wi;
-                    wroot((long long)i * n1, 32, wr, wi);
+                    wroot((long long)i * n1, 32, wr, wi, conj);
                     const long double v = (m == 0) ? (ks < 2 ? wr : -wi) : (ks < 2 ? wi : wr);
                     split_store<S>((float)v, terms, &A[(((size_t)(m * 4 + ks) * terms) * 64 + lane) * 8 + e], 64 * 8);
                 }
@@ -368,7 +367,7 @@ template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int 
             for (int e = 0; e < 8; e++) {
                 const int i = lane & 31, n2 = 8 * (lane >> 5) + e;
                 long double wr, wi;
-                wroot((long long)(i & 15) * n2, 16, wr, wi);
+                wroot((long long)(i & 15) * n2, 16, wr, wi, conj);
                 const long double v = (i < 16) ? (ks == 0 ? wr : -wi) : (ks == 0 ? wi : wr);
                 split_store<S>((float)v, terms, &Cm[(((size_t)ks * terms) * 64 + lane) * 8 + e], 64 * 8);
             }
@@ -379,17 +378,17 @@ template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int 
 }
 }  // namespace
 
-std::vector<unsigned char> mfma512_operand_table(int prec) {
+std::vector<unsigned char> mfma512_operand_table(int prec, bool conj) {
     std::vector<unsigned char> bytes(512 * 2 * sizeof(float));
     float *w = reinterpret_cast<float *>(bytes.data());
     for (int r = 0; r < 512; r++) {
         long double re, im;
-        wroot(r, 512, re, im);
+        wroot(r, 512, re, im, conj);
         w[2 * r] = (float)re;
         w[2 * r + 1] = (float)im;
     }
-    if (prec == kSplitF16) fill_operands<_Float16>(bytes, 2);
-    else fill_operands<__bf16>(bytes, 3);
+    if (prec == kSplitF16) fill_operands<_Float16>(bytes, 2, conj);
+    else fill_operands<__bf16>(bytes, 3, conj);
     return bytes;
 }
 

@@ -107,8 +107,9 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
 fft_plan::~fft_plan() {
     for (auto &kv : tw_)
         if (kv.second.dev) (void)hipFree(kv.second.dev);
-    for (auto &t : mfma_tw_)
-        if (t.dev) (void)hipFree(t.dev);
+    for (auto &tp : mfma_tw_)
+        for (auto &t : tp)
+            if (t.dev) (void)hipFree(t.dev);
 }
 
 double fft_plan::scale_factor(int scaling) const {
@@ -551,15 +552,22 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         const bool mfma = precision_ != kPrecPlain && !dbl && p.n == 512 && p.mode == kModeC2C && !p.identity &&
                           !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
         if (mfma) {
-            // the split-precision form: W32, W16 and W512 tables, 16 lines per block
-            const int pi = precision_ == kSplitF16 ? 0 : 1;
-            if (!mfma_tw_[pi].dev) {
-                std::vector<unsigned char> t = mfma512_operand_table(precision_);
-                mfma_tw_[pi].bytes = t.size();
-                check_hip(hipMalloc(&mfma_tw_[pi].dev, t.size()), "hipMalloc(twiddles)");
-                check_hip(hipMemcpy(mfma_tw_[pi].dev, t.data(), t.size(), hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+            // the split-precision form: W512 and the split W32 / W16 operand tables, 16-line
+            // tiles; the backward pass multiplies by the conjugate matrices directly (no
+            // conjugation of the data on load and store)
+            const int pi = precision_ == kSplitF16 ? 0 : 1, di = dir > 0 ? 1 : 0;
+            {
+                std::lock_guard<std::mutex> lock(mtx_);
+                twiddles &mt = mfma_tw_[pi][di];
+                if (!mt.dev) {
+                    std::vector<unsigned char> t = mfma512_operand_table(precision_, di == 1);
+                    mt.bytes = t.size();
+                    check_hip(hipMalloc(&mt.dev, t.size()), "hipMalloc(twiddles)");
+                    check_hip(hipMemcpy(mt.dev, t.data(), t.size(), hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+                }
+                a.tw = mt.dev;
             }
-            a.tw = mfma_tw_[pi].dev;
+            a.conj = 0;
             check_hip(launch_mfma512(precision_, a, ((long long)a.nlines + 15) / 16, stream_), "MFMA FFT pass");
         } else if (p.fast) {
             check_hip(p.fast->launch(a, nblocks, stream_), "fast FFT pass");

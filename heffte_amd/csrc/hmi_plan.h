@@ -113,7 +113,7 @@ class fft_plan {
     hipStream_t stream_ = nullptr;
     int precision_ = kPrecPlain;
     std::mutex mtx_;
-    twiddles mfma_tw_[2] = {};  // split FP16, split BF16
+    twiddles mfma_tw_[2][2] = {};  // [split FP16, split BF16][forward, backward]
     std::map<int, exec_plan> plans_;
     std::map<long long, twiddles> tw_;
 };

@@ -68,6 +68,23 @@ __device__ __forceinline__ void split8(const float (&x)[8], typename split_trait
     }
 }
 
+// FP16 hi/lo split of 8 values times a power-of-two scale: per pair, v_fma_mix forms
+// hi = f16(x s) and lo = f16(x s - hi) with the scale folded into the multiply
+__device__ __forceinline__ void split8_f16_scaled(const float (&x)[8], float s, half8_t (&out)[3]) {
+    unsigned hi[4], lo[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        asm volatile(
+            "v_fma_mixlo_f16 %0, %2, %4, 0\n\t"
+            "v_fma_mixhi_f16 %0, %3, %4, 0\n\t"
+            "v_fma_mixlo_f16 %1, %2, %4, -%0 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %1, %3, %4, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+            : "=&v"(hi[q]), "=&v"(lo[q])
+            : "v"(x[2 * q]), "v"(x[2 * q + 1]), "v"(s));
+    __builtin_memcpy(&out[0], hi, sizeof(hi));
+    __builtin_memcpy(&out[1], lo, sizeof(lo));
+}
+
 // acc += A * B through the split products
 template <int PREC>
 __device__ __forceinline__ f32x16_t split_mma(const typename split_traits<PREC>::vec (&a)[3],
@@ -247,8 +264,13 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
                     const float2 x = rowp[16 * half + q];
-                    re[q] = x.x * s;
-                    im[q] = x.y * s;
+                    if constexpr (PREC == kSplitF16) {
+                        re[q] = x.x;
+                        im[q] = x.y;
+                    } else {
+                        re[q] = x.x * s;
+                        im[q] = x.y * s;
+                    }
                 }
 #pragma unroll
                 for (int part = 0; part < 2; part++) {
@@ -256,7 +278,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                     vec A[3], B[3];
 #pragma unroll
                     for (int tt = 0; tt < TERMS; tt++) A[tt] = opm[(ks * TERMS + tt) * 64];
-                    split8<PREC>(part == 0 ? re : im, B);
+                    if constexpr (PREC == kSplitF16) split8_f16_scaled(part == 0 ? re : im, s, B);
+                    else split8<PREC>(part == 0 ? re : im, B);
                     acc = split_mma<PREC>(A, B, acc);
                 }
             }
@@ -296,7 +319,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 #pragma unroll
                 for (int tt = 0; tt < TERMS; tt++)
                     A[tt] = kTablesInLds ? opCs[(ks * TERMS + tt) * 64 + lane] : opC[(ks * TERMS + tt) * 64 + lane];
-                split8<PREC>(ks == 0 ? zr : zi, B);
+                if constexpr (PREC == kSplitF16) split8_f16_scaled(ks == 0 ? zr : zi, 1.0f, B);
+                else split8<PREC>(ks == 0 ? zr : zi, B);
                 acc = split_mma<PREC>(A, B, acc);
             }
             // store: lane holds X[k1 + 32 k2], k2 = (e & 3) + 8 (e >> 2) + 4 h, e < 8

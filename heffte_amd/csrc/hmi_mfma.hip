@@ -141,10 +141,11 @@ __device__ __forceinline__ long long tile_line_off(unsigned tile, int l, const p
 // the rows over the banks).
 //
 // Persistent form: one 1024-thread block per CU walks the tiles blockIdx.x, blockIdx.x +
-// gridDim.x, ...  The split DFT operands (and, for FP16, W512) are copied to LDS once, and
-// the global loads of the next tile are in flight while the current tile runs step A.  Both
-// tile layouts, the tables and the per-line maxima (by tile parity: one partial per wave and
-// line) take 163,728 B (FP16) or 163,840 B (BF16) of LDS: one block per CU.
+// gridDim.x, ...  FP16 keeps each wave's step-A operands in registers and the step-C operands
+// and W512 in LDS; BF16 keeps its step-A operands in LDS and reads the rest from the cached
+// table.  The global loads of the next tile are in flight while the current tile runs step A.
+// Both tile layouts, the tables and the per-line maxima (by tile parity: one partial per wave
+// and line) take 147,344 B (FP16) or 163,840 B (BF16) of LDS: one block per CU.
 // Per tile: issue loads(next) | step A | barrier | Y -> LDS, next -> LDS | barrier | step C, store.
 template <int PREC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
@@ -159,8 +160,10 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     constexpr int PAD = kTablesInLds ? 1 : 0;
     constexpr int RA = 33, RLA = 16 * RA + PAD;
     constexpr int RC = 17, RLC = 32 * RC + PAD;
+    // FP16 keeps each wave's step-A operands in registers; BF16 (more terms) reads them from LDS
     constexpr size_t kA = 0, kC = kA + sizeof(float2) * T * RLA, kOpA = kC + sizeof(float2) * T * RLC;
-    constexpr size_t kMax = kOpA + sizeof(vec) * 2 * 4 * TERMS * 64, kW = kMax + sizeof(float) * 2 * T * T;
+    constexpr size_t kMax = kOpA + (kTablesInLds ? 0 : sizeof(vec) * 2 * 4 * TERMS * 64);
+    constexpr size_t kW = kMax + sizeof(float) * 2 * T * T;
     constexpr size_t kOpC = kW + (kTablesInLds ? sizeof(float2) * 466 : 0);
     constexpr size_t kBytes = kOpC + (kTablesInLds ? sizeof(vec) * 2 * TERMS * 64 : 0);
     __shared__ __align__(16) unsigned char lds[kBytes];
@@ -178,7 +181,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;
     {
         const int tid = threadIdx.x;
-        for (int i = tid; i < 2 * 4 * TERMS * 64; i += 1024) opAs[i] = opA[i];
+        if constexpr (!kTablesInLds)
+            for (int i = tid; i < 2 * 4 * TERMS * 64; i += 1024) opAs[i] = opA[i];
         if constexpr (kTablesInLds) {
             if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
             if (tid < 466) w512s[tid] = w512[tid];
@@ -239,6 +243,15 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         return m;
     };
 
+    // FP16: the wave's step-A operands stay in registers for the whole launch
+    vec ra[4][TERMS];
+    if constexpr (kTablesInLds) {
+        const int lane = threadIdx.x & 63, m = (threadIdx.x >> 6) & 1;
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int tt = 0; tt < TERMS; tt++) ra[ks][tt] = opA[((m * 4 + ks) * TERMS + tt) * 64 + lane];
+    }
     // one tile t: loads of tile t + G into xn | step A | barrier | Y -> LDS, tile t + G -> LDS |
     // barrier | step C, store
     auto run_tile = [&](unsigned tile, int par, float2 (&xn)[E]) {
@@ -277,7 +290,10 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                     const int ks = half + 2 * part;
                     vec A[3], B[3];
 #pragma unroll
-                    for (int tt = 0; tt < TERMS; tt++) A[tt] = opm[(ks * TERMS + tt) * 64];
+                    for (int tt = 0; tt < TERMS; tt++) {
+                        if constexpr (kTablesInLds) A[tt] = ra[ks][tt];
+                        else A[tt] = opm[(ks * TERMS + tt) * 64];
+                    }
                     if constexpr (PREC == kSplitF16) split8_f16_scaled(part == 0 ? re : im, s, B);
                     else split8<PREC>(part == 0 ? re : im, B);
                     acc = split_mma<PREC>(A, B, acc);

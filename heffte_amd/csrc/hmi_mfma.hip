@@ -212,16 +212,15 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     // sign, step-A layout (x[16 n1 + n2] at row n2, column n1; pos = t + 64 e has
     // n2 = t & 15, n1 = (t >> 4) + 4 e), per-line maximum: a full line per wave (along the axis)
     // or lanes l, l+16, l+32, l+48 of every wave (column-fast), one partial per wave
-    auto deposit = [&](unsigned tile, float2 (&x)[E], int par) {
+    auto deposit = [&](float2 (&x)[E], int par) {
         const int tid = thread();
         const int lane = tid & 63, wave = tid >> 6;
         const int l = lcf ? (tid & 15) : (tid >> 6), t = lcf ? (tid >> 4) : (tid & 63);
-        const bool valid = tile * T + l < nlines;
+        // a line past the end holds line 0's values: its results are never stored
         float2 *row = &ldsA[l * RLA + (t & 15) * RA + (t >> 4)];
         float m = 0.f;
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            if (!valid) x[e] = make_float2(0.f, 0.f);
             m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
             row[4 * e] = x[e];
         }
@@ -310,7 +309,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 #pragma unroll
             for (int e = 0; e < 16; e++) yp[2 * RC * ((e & 3) + 8 * (e >> 2))] = acc[e];  // k1 = that + 4 h
         }
-        if (has_next) deposit(next, xn, par ^ 1);
+        if (has_next) deposit(xn, par ^ 1);
         __syncthreads();
 
         // ---- twiddle + step C: wave -> 32 columns (k1, line)
@@ -359,7 +358,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     {
         float2 x[E];
         issue(tile, x);
-        deposit(tile, x, 0);
+        deposit(x, 0);
     }
     __syncthreads();
     for (int it = 0; tile < ntiles; it++, tile += gridDim.x) {

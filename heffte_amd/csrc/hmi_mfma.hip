@@ -141,11 +141,11 @@ __device__ __forceinline__ long long tile_line_off(unsigned tile, int l, const p
 // the rows over the banks).
 //
 // Persistent form: one 1024-thread block per CU walks the tiles blockIdx.x, blockIdx.x +
-// gridDim.x, ...  FP16 keeps each wave's step-A operands in registers and the step-C operands
-// and W512 in LDS; BF16 keeps its step-A operands in LDS and reads the rest from the cached
-// table.  The global loads of the next tile are in flight while the current tile runs step A.
-// Both tile layouts, the tables and the per-line maxima (by tile parity: one partial per wave
-// and line) take 147,344 B (FP16) or 163,840 B (BF16) of LDS: one block per CU.
+// gridDim.x, ...  The step-C operands and W512 are copied to LDS once; FP16 keeps each wave's
+// step-A operands in registers, BF16 reads them from the cached table.  The global loads of the
+// next tile are in flight while the current tile runs step A.  Both tile layouts, the tables
+// and the per-line maxima (by tile parity: one partial per wave and line) take 147,344 B (FP16)
+// or 149,392 B (BF16) of LDS: one block per CU.
 // Per tile: issue loads(next) | step A | barrier | Y -> LDS, next -> LDS | barrier | step C, store.
 template <int PREC>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
@@ -154,22 +154,18 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     using S = split_traits<PREC>;
     using vec = typename S::vec;
     constexpr int TERMS = S::terms;
-    // The three-term BF16 operands leave no LDS for W512 and the step-C operands (read from
-    // the cached table instead) nor for the one-element pad between lines.
-    constexpr bool kTablesInLds = (TERMS == 2);
-    constexpr int PAD = kTablesInLds ? 1 : 0;
-    constexpr int RA = 33, RLA = 16 * RA + PAD;
-    constexpr int RC = 17, RLC = 32 * RC + PAD;
-    // FP16 keeps each wave's step-A operands in registers; BF16 (more terms) reads them from LDS
-    constexpr size_t kA = 0, kC = kA + sizeof(float2) * T * RLA, kOpA = kC + sizeof(float2) * T * RLC;
-    constexpr size_t kMax = kOpA + (kTablesInLds ? 0 : sizeof(vec) * 2 * 4 * TERMS * 64);
+    // FP16 keeps each wave's step-A operands in registers; BF16 (three terms, too many
+    // registers) reads them from the cached table every tile
+    constexpr bool kOpARegs = (TERMS == 2);
+    constexpr int RA = 33, RLA = 16 * RA + 1;
+    constexpr int RC = 17, RLC = 32 * RC + 1;
+    constexpr size_t kA = 0, kC = kA + sizeof(float2) * T * RLA, kMax = kC + sizeof(float2) * T * RLC;
     constexpr size_t kW = kMax + sizeof(float) * 2 * T * T;
-    constexpr size_t kOpC = kW + (kTablesInLds ? sizeof(float2) * 466 : 0);
-    constexpr size_t kBytes = kOpC + (kTablesInLds ? sizeof(vec) * 2 * TERMS * 64 : 0);
+    constexpr size_t kOpC = kW + sizeof(float2) * 466;
+    constexpr size_t kBytes = kOpC + sizeof(vec) * 2 * TERMS * 64;
     __shared__ __align__(16) unsigned char lds[kBytes];
     float2 *ldsA = reinterpret_cast<float2 *>(lds + kA);
     float2 *ldsC = reinterpret_cast<float2 *>(lds + kC);
-    vec *opAs = reinterpret_cast<vec *>(lds + kOpA);
     float(*pmax)[T][T] = reinterpret_cast<float(*)[T][T]>(lds + kMax);
     float2 *w512s = reinterpret_cast<float2 *>(lds + kW);  // W512^r for r = n2 k1 <= 15 * 31
     vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
@@ -181,12 +177,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     const vec *__restrict__ opC = opA + 2 * 4 * TERMS * 64;
     {
         const int tid = threadIdx.x;
-        if constexpr (!kTablesInLds)
-            for (int i = tid; i < 2 * 4 * TERMS * 64; i += 1024) opAs[i] = opA[i];
-        if constexpr (kTablesInLds) {
-            if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
-            if (tid < 466) w512s[tid] = w512[tid];
-        }
+        for (int i = tid; i < 2 * TERMS * 64; i += 1024) opCs[i] = opC[i];
+        if (tid < 466) w512s[tid] = w512[tid];
     }
     float *ldsCf = reinterpret_cast<float *>(ldsC);
     const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
@@ -244,7 +236,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 
     // FP16: the wave's step-A operands stay in registers for the whole launch
     vec ra[4][TERMS];
-    if constexpr (kTablesInLds) {
+    if constexpr (kOpARegs) {
         const int lane = threadIdx.x & 63, m = (threadIdx.x >> 6) & 1;
 #pragma unroll
         for (int ks = 0; ks < 4; ks++)
@@ -269,7 +261,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
             (void)frexpf(line_max(par, line), &ex);
             const float s = ldexpf(1.0f, -ex);
             const float2 *rowp = &ldsA[line * RLA + n2 * RA + 8 * h];
-            const vec *opm = &opAs[m * 4 * TERMS * 64 + lane];
+            const vec *opm = &opA[m * 4 * TERMS * 64 + lane];
 #pragma unroll
             for (int half = 0; half < 2; half++) {
                 float re[8], im[8];
@@ -290,7 +282,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                     vec A[3], B[3];
 #pragma unroll
                     for (int tt = 0; tt < TERMS; tt++) {
-                        if constexpr (kTablesInLds) A[tt] = ra[ks][tt];
+                        if constexpr (kOpARegs) A[tt] = ra[ks][tt];
                         else A[tt] = opm[(ks * TERMS + tt) * 64];
                     }
                     if constexpr (PREC == kSplitF16) split8_f16_scaled(part == 0 ? re : im, s, B);
@@ -324,7 +316,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
             for (int e = 0; e < 8; e++) {
                 const float2 y = yp[e];
                 // W512^{n2 k1}, n2 = 8 h + e
-                const float2 w = kTablesInLds ? w512s[(8 * h + e) * k1] : w512[(8 * h + e) * k1];
+                const float2 w = w512s[(8 * h + e) * k1];
                 zr[e] = y.x * w.x - y.y * w.y;
                 zi[e] = y.x * w.y + y.y * w.x;
             }
@@ -333,7 +325,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                 vec A[3], B[3];
 #pragma unroll
                 for (int tt = 0; tt < TERMS; tt++)
-                    A[tt] = kTablesInLds ? opCs[(ks * TERMS + tt) * 64 + lane] : opC[(ks * TERMS + tt) * 64 + lane];
+                    A[tt] = opCs[(ks * TERMS + tt) * 64 + lane];
                 if constexpr (PREC == kSplitF16) split8_f16_scaled(ks == 0 ? zr : zi, 1.0f, B);
                 else split8<PREC>(ks == 0 ? zr : zi, B);
                 acc = split_mma<PREC>(A, B, acc);

@@ -145,3 +145,14 @@ def test_proc_setup_and_split_world(hmi):
                     assert (i, j, k) not in covered
                     covered.add((i, j, k))
     assert len(covered) == w2.count()
+
+
+def test_length_beyond_single_pass_capacity_is_rejected(hmi):
+    """A line that does not fit the 160 KiB LDS of one workgroup (2 n element sizes) is refused
+    at plan creation with a message, not run."""
+    ok = hmi.box3d([0, 0, 0], [10239, 0, 0])
+    hmi.fft3d(hmi.backend.mi355, ok, ok)
+    big = hmi.box3d([0, 0, 0], [10240 + 1, 0, 0])  # 10242 = 2 * 3 * 3 * 569
+    with pytest.raises(hmi.heffte_input_error) as e:
+        hmi.fft3d(hmi.backend.mi355, big, big)
+    assert "LDS" in str(e.value)

@@ -115,6 +115,18 @@ def test_fast_and_generic_lengths(hmi, cuda, oracle, dtype, size):
             assert err <= fc.TOL[dtype], (size, inorder, outorder, in_place, err)
 
 
+@pytest.mark.parametrize("dtype,size", [("complex64", (8192, 2, 2)), ("complex64", (10240, 1, 2)),
+                                        ("complex128", (5120, 2, 1)), ("complex64", (1021, 3, 2)),
+                                        ("complex128", (2, 1021, 3)), ("complex64", (4096, 4, 3))])
+def test_long_and_prime_lengths(hmi, cuda, oracle, dtype, size):
+    """Lengths up to the single-pass LDS capacity (2 n element sizes <= 160 KiB: 10240 complex<float>,
+    5120 complex<double>) and a large prime (1021: direct DFT stage of the runtime plan)."""
+    for inorder, outorder in [((0, 1, 2), (0, 1, 2)), ((0, 1, 2), (2, 0, 1))]:
+        x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=8)
+        ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+        assert fc.rel_l2(y, ref) <= fc.TOL[dtype], (size, inorder, outorder)
+
+
 @pytest.mark.parametrize("rdt,cdt", [(np.float32, np.complex64), (np.float64, np.complex128)])
 def test_real_input_and_real_output(hmi, cuda, oracle, rdt, cdt):
     """fft3d::forward(real in, complex out) and backward(complex in, real out),

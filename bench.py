@@ -42,7 +42,7 @@ def parse():
     p.add_argument("--no-batched", action="store_true", help="skip the 1000-transform mixed-size batch leg")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA forms")
     p.add_argument("--no-secondary", action="store_true",
-                   help="skip the r2c 1024x1024x512 and DCT 256^3 legs")
+                   help="skip the r2c 1024x1024x512, DCT 256^3 and 512^3 reshape legs")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a plain 1 GiB device copy (reference for the HBM byte counters)")
     return p.parse_args()
@@ -246,6 +246,25 @@ def precision_leg(heffte, torch, dev, plan, n):
     return res
 
 
+def reshape_leg(heffte, torch, dev, steps):
+    """The bit-exact in-node reshape alone (heFFTe's transpose between pencil orders, here
+    permute_kernel): 512^3 complex<float> from order (0,1,2) to (1,2,0) and to (2,1,0), out of
+    place.  In the fused path this permutation costs no pass of its own; timed here as what an
+    unfused reshape would add."""
+    n = 512
+    box = heffte.box3d([0, 0, 0], [n - 1] * 3)
+    x = torch.randn(n ** 3, dtype=torch.complex64, device=dev)
+    y = torch.empty_like(x)
+    res = {}
+    for order in ((1, 2, 0), (2, 1, 0)):
+        ms = _timed(torch, lambda: heffte.reshape(box, order, x, y), steps)
+        res["to_" + "".join(map(str, order))] = {"ms": round(ms, 4),
+                                                 "hbm_GBps": round(2 * 8 * n ** 3 / (ms * 1e-3) / 1e9, 1)}
+    del x, y
+    torch.cuda.empty_cache()
+    return res
+
+
 def dct_leg(heffte, torch, dev, steps):
     """rtransform<cos> (DCT-II forward, DCT-III backward) of a 256^3 double box, order (0,1,2),
     out of place: three real-to-real passes per direction."""
@@ -436,6 +455,7 @@ def main():
         torch.cuda.empty_cache()
         out["r2c_1024x1024x512_f32"] = r2c_leg(heffte, torch, dev, max(3, args.steps // 2))
         out["dct_256_f64"] = dct_leg(heffte, torch, dev, max(3, args.steps // 2))
+        out["reshape_512_c64"] = reshape_leg(heffte, torch, dev, max(3, args.steps // 2))
         data = ws = None
 
     if not args.no_cpu:

@@ -628,8 +628,10 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
     X(float, 256, 16, 16, 4, 16, 16)             \
     X(float, 512, 8, 16, 3, 8, 8, 8)             \
     X(float, 512, 8, 4, 3, 8, 8, 8)              \
+    X(float, 1024, 16, 16, 4, 16, 8, 8)          \
     X(float, 1024, 16, 8, 4, 16, 8, 8)           \
     X(float, 1024, 16, 4, 4, 16, 8, 8)           \
+    X(float, 2048, 16, 8, 4, 8, 16, 16)          \
     X(float, 2048, 16, 4, 4, 8, 16, 16)          \
     X(float, 2048, 16, 2, 4, 8, 16, 16)          \
     X(float, 4096, 16, 4, 4, 16, 16, 16)         \
@@ -655,8 +657,10 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
     X(double, 256, 16, 8, 4, 16, 16)             \
     X(double, 512, 8, 8, 3, 8, 8, 8)             \
     X(double, 512, 8, 4, 3, 8, 8, 8)             \
+    X(double, 1024, 16, 8, 4, 16, 8, 8)          \
     X(double, 1024, 16, 4, 4, 16, 8, 8)          \
     X(double, 1024, 16, 2, 4, 16, 8, 8)          \
+    X(double, 2048, 16, 4, 4, 8, 16, 16)         \
     X(double, 2048, 16, 2, 4, 8, 16, 16)         \
     X(double, 4096, 16, 2, 4, 16, 16, 16)        \
     X(double, 4096, 16, 1, 4, 16, 16, 16)

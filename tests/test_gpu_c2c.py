@@ -127,6 +127,17 @@ def test_long_and_prime_lengths(hmi, cuda, oracle, dtype, size):
         assert fc.rel_l2(y, ref) <= fc.TOL[dtype], (size, inorder, outorder)
 
 
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("size", [(1024, 20, 16), (2048, 12, 8), (16, 1024, 3)])
+def test_wide_tiles_of_long_lines(hmi, cuda, oracle, dtype, size):
+    """Passes over 1024- and 2048-point lines whose load or store walks the column axis use the
+    widest tiles (16 x 8 B or 8 x 16 B column chunks where LDS allows)."""
+    for inorder, outorder in [((0, 1, 2), (1, 2, 0)), ((2, 1, 0), (0, 1, 2)), ((1, 0, 2), (2, 0, 1))]:
+        x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=6)
+        ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+        assert fc.rel_l2(y, ref) <= fc.TOL[dtype], (size, inorder, outorder)
+
+
 @pytest.mark.parametrize("rdt,cdt", [(np.float32, np.complex64), (np.float64, np.complex128)])
 def test_real_input_and_real_output(hmi, cuda, oracle, rdt, cdt):
     """fft3d::forward(real in, complex out) and backward(complex in, real out),

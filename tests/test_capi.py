@@ -156,3 +156,29 @@ def test_length_beyond_single_pass_capacity_is_rejected(hmi):
     with pytest.raises(hmi.heffte_input_error) as e:
         hmi.fft3d(hmi.backend.mi355, big, big)
     assert "LDS" in str(e.value)
+
+
+def test_batch_plan_creation_rules(hmi):
+    """Batch plans are created on the host: sizes and offsets are checked, extents reported."""
+    p = hmi.batch_plan([(16, 20, 24), (7, 1, 3)])
+    assert p.size_input() == 16 * 20 * 24 + 21 and p.size_output() == p.size_input()
+    q = hmi.batch_plan([(4, 4, 4), (2, 2, 2)], in_offsets=[100, 0], out_offsets=[0, 64])
+    assert q.size_input() == 164 and q.size_output() == 72
+    for bad in ([], [(0, 4, 4)], [(4, -1, 4)]):
+        with pytest.raises(hmi.heffte_input_error):
+            hmi.batch_plan(bad)
+    with pytest.raises(hmi.heffte_input_error):
+        hmi.batch_plan([(4, 4, 4)], in_offsets=[-1])
+    with pytest.raises(hmi.heffte_input_error):
+        hmi.batch_plan([(20000, 2, 2)])  # a side beyond the single-pass LDS capacity
+
+
+def test_batch_c_abi_rejects_bad_handles(hmi):
+    import ctypes
+    lib = hmi.lib
+    assert lib.heffte_mi355_batch_set_graph(None, 1) != 0
+    assert lib.heffte_mi355_batch_set_autotune(None, 1) != 0
+    assert lib.heffte_mi355_batch_size_input(None) == -1
+    h = ctypes.c_void_p()
+    assert lib.heffte_mi355_batch_create(1, None, None, None, ctypes.byref(h)) != 0
+    assert "size" in hmi.last_error()

@@ -265,6 +265,52 @@ def reshape_leg(heffte, torch, dev, steps):
     return res
 
 
+def convolve_leg(heffte, torch, dev, steps):
+    """Convolution through the transform (benchmarks/convolution.cpp:86-95: forward, pointwise
+    product, backward) on 512^3 complex<float>, inbox order (0,1,2), outbox (1,2,0).  Unfused =
+    the reference's sequence through the plan with the product as a separate torch kernel;
+    fused = plan.convolve (product folded into the first backward pass).  'self' squares one
+    spectrum as the reference benchmark does; 'pair' convolves two arrays."""
+    n = 512
+    inbox = heffte.box3d([0, 0, 0], [n - 1] * 3, [0, 1, 2])
+    outbox = heffte.box3d([0, 0, 0], [n - 1] * 3, [1, 2, 0])
+    plan = heffte.fft3d(heffte.backend.mi355, inbox, outbox)
+    x = plane_waves(torch, dev, (n, n, n), 3, torch.complex64)
+    y = plane_waves(torch, dev, (n, n, n), 4, torch.complex64)
+    out, fy = torch.empty_like(x), torch.empty_like(x)
+    ws = torch.empty(max(1, plan.size_convolve_workspace()), dtype=torch.complex64, device=dev)
+    wsf = ws[:max(1, plan.size_workspace())]
+
+    def self_unfused():
+        plan.forward_buffered(x, out, wsf, heffte.scale.full)
+        out.mul_(out)
+        plan.backward_buffered(out, out, wsf)
+
+    def pair_unfused():
+        plan.forward_buffered(x, out, wsf)
+        plan.forward_buffered(y, fy, wsf)
+        out.mul_(fy)
+        plan.backward_buffered(out, out, wsf, heffte.scale.full)
+
+    res = {"workload": "convolution 512^3 complex<float>, order (0,1,2) -> (1,2,0), plane waves + noise"}
+    ms_su = _timed(torch, self_unfused, steps)
+    ref = out.clone()
+    ms_sf = _timed(torch, lambda: plan.convolve(x, x, out, ws, heffte.scale.none), steps)
+    # the reference scales the forward transform (1/N twice through the square); same work
+    d_self = (torch.linalg.vector_norm(out / float(n ** 3) ** 2 - ref) / torch.linalg.vector_norm(ref)).item()
+    ms_pu = _timed(torch, pair_unfused, steps)
+    ref.copy_(out)
+    ms_pf = _timed(torch, lambda: plan.convolve(x, y, out, ws, heffte.scale.full), steps)
+    d_pair = (torch.linalg.vector_norm(out - ref) / torch.linalg.vector_norm(ref)).item()
+    res.update({"self_ms_unfused": round(ms_su, 4), "self_ms_fused": round(ms_sf, 4),
+                "self_speedup": round(ms_su / ms_sf, 3), "self_rel_diff": d_self,
+                "pair_ms_unfused": round(ms_pu, 4), "pair_ms_fused": round(ms_pf, 4),
+                "pair_speedup": round(ms_pu / ms_pf, 3), "pair_rel_diff": d_pair})
+    del x, y, out, fy, ws, wsf, ref
+    torch.cuda.empty_cache()
+    return res
+
+
 def dct_leg(heffte, torch, dev, steps):
     """rtransform<cos> (DCT-II forward, DCT-III backward) of a 256^3 double box, order (0,1,2),
     out of place: three real-to-real passes per direction."""
@@ -456,6 +502,7 @@ def main():
         out["r2c_1024x1024x512_f32"] = r2c_leg(heffte, torch, dev, max(3, args.steps // 2))
         out["dct_256_f64"] = dct_leg(heffte, torch, dev, max(3, args.steps // 2))
         out["reshape_512_c64"] = reshape_leg(heffte, torch, dev, max(3, args.steps // 2))
+        out["convolve_512_c64"] = convolve_leg(heffte, torch, dev, max(3, args.steps // 2))
         data = ws = None
 
     if not args.no_cpu:

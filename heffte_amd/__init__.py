@@ -63,6 +63,12 @@ def _load():
         fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, ctypes.c_int]
         fn.restype = ctypes.c_int
     lib.heffte_mi355_reshape.argtypes = [ctypes.c_int, LL3, LL3, I3, I3, ctypes.c_int, vp, vp, vp]
+    lib.heffte_mi355_size_convolve_workspace.argtypes = [plan_p]
+    lib.heffte_mi355_size_convolve_workspace.restype = ctypes.c_longlong
+    for f in ("convolve_c2c", "convolve_z2z"):
+        fn = getattr(lib, "heffte_mi355_" + f)
+        fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
+        fn.restype = ctypes.c_int
     lib.heffte_mi355_batch_create.argtypes = [ctypes.c_int, I3, LL3, LL3, ctypes.POINTER(plan_p)]
     lib.heffte_mi355_batch_destroy.argtypes = [plan_p]
     lib.heffte_mi355_batch_set_stream.argtypes = [plan_p, vp]
@@ -95,6 +101,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_set_graph", "heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output",
     "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
     "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
+    "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
 )
 
 
@@ -254,6 +261,31 @@ class fft_plan:
 
     def backward_buffered(self, inarray, outarray, workspace, scaling=scale.none, batch=1):
         self._call("backward", inarray, outarray, workspace, scaling, batch)
+
+    def size_convolve_workspace(self):
+        return lib.heffte_mi355_size_convolve_workspace(self._h)
+
+    def convolve(self, x, y, outarray, workspace=None, scaling=scale.full, batch=1):
+        """outarray = backward(forward(x) * forward(y)) with the backward scaling; scale.full
+        gives the circular convolution of x and y (the sequence of benchmarks/convolution.cpp:86-95
+        in one call, the product folded into the first backward pass).  c2c plans only;
+        x, y and outarray are complex arrays in the inbox layout, outarray may alias x or y."""
+        if self.kind != kind.c2c:
+            raise heffte_input_error("convolve() needs a c2c plan")
+        if scaling not in (0, 1, 2):
+            raise heffte_input_error("convolve() called with invalid scaling")
+        names = {_dtype_name(x), _dtype_name(y), _dtype_name(outarray)}
+        if len(names) != 1 or names.copy().pop() not in ("complex64", "complex128"):
+            raise heffte_input_error(f"convolve() called with incompatible types {sorted(names)}")
+        key = "c2c" if _dtype_name(x) == "complex64" else "z2z"
+        n = batch * self.size_inbox()
+        if x.numel() < n or y.numel() < n or outarray.numel() < n:
+            raise heffte_input_error("convolve() called with invalid array size")
+        if workspace is not None and workspace.numel() * workspace.element_size() < \
+                batch * self.size_convolve_workspace() * (8 if key == "c2c" else 16):
+            raise heffte_input_error("convolve() called with insufficient workspace size")
+        fn = getattr(lib, f"heffte_mi355_convolve_{key}")
+        _check(fn(self._h, batch, _ptr(x), _ptr(y), _ptr(outarray), _ptr(workspace), scaling), "convolve")
 
 
 def _check_backend(backend_tag):

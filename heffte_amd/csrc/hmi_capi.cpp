@@ -92,6 +92,14 @@ int run_checked(heffte_mi355_plan plan, bool r2r, int dir, bool dbl, bool in_rea
     return run(plan, dir, dbl, in_real, out_real, batch, in, out, ws, scale);
 }
 
+int convolve(heffte_mi355_plan plan, bool dbl, int batch, const void *x, const void *y, void *out, void *ws,
+             int scale) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    if (plan->impl->get_kind() != hmi::kind::c2c) return fail(HMI_ERROR_INVALID_ARGUMENT, "convolution needs a c2c plan");
+    if (scale < 0 || scale > 2) return fail(HMI_ERROR_INVALID_ARGUMENT, "scale must be 0, 1 or 2");
+    return guarded([&] { plan->impl->convolve(dbl, batch, x, y, out, ws, scale); });
+}
+
 }  // namespace
 
 extern "C" {
@@ -165,6 +173,10 @@ int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision) {
     return HMI_SUCCESS;
 }
 
+long long heffte_mi355_size_convolve_workspace(heffte_mi355_plan plan) {
+    return valid(plan) && plan->impl->get_kind() == hmi::kind::c2c ? plan->impl->convolve_workspace() : -1;
+}
+
 int heffte_mi355_forward_c2c(heffte_mi355_plan p, int b, void const *in, void *out, void *ws, int s) {
     return run_checked(p, false, -1, false, false, false, b, in, out, ws, s);
 }
@@ -188,6 +200,12 @@ int heffte_mi355_backward_c2s(heffte_mi355_plan p, int b, void const *in, float 
 }
 int heffte_mi355_backward_z2d(heffte_mi355_plan p, int b, void const *in, double *out, void *ws, int s) {
     return run_checked(p, false, 1, true, false, true, b, in, out, ws, s);
+}
+int heffte_mi355_convolve_c2c(heffte_mi355_plan p, int b, void const *x, void const *y, void *out, void *ws, int s) {
+    return convolve(p, false, b, x, y, out, ws, s);
+}
+int heffte_mi355_convolve_z2z(heffte_mi355_plan p, int b, void const *x, void const *y, void *out, void *ws, int s) {
+    return convolve(p, true, b, x, y, out, ws, s);
 }
 int heffte_mi355_forward_s2s(heffte_mi355_plan p, int b, float const *in, float *out, void *ws, int s) {
     return run_checked(p, true, -1, false, true, true, b, in, out, ws, s);

@@ -213,10 +213,21 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             }
         } else {
             const C *__restrict__ src = static_cast<const C *>(a.in);
+            if (a.mul) {
+                // pointwise product with a second array of the same layout (fused convolution)
+                const C *__restrict__ mul = static_cast<const C *>(a.mul);
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                C x = valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
-                v[e] = cmk<C>(x.x, sg * x.y);
+                for (int e = 0; e < E; e++) {
+                    const long long off = base + (long long)(t + P * e) * a.in_sa;
+                    C x = valid ? cmul(ldg<NT>(&src[off]), ldg<NT>(&mul[off])) : cmk<C>(R(0), R(0));
+                    v[e] = cmk<C>(x.x, sg * x.y);
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    C x = valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
+                    v[e] = cmk<C>(x.x, sg * x.y);
+                }
             }
         }
     }
@@ -380,6 +391,7 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
                 x.y *= -sg;
             } else {
                 x = static_cast<const C *>(a.in)[off];
+                if (a.mul) x = cmul(x, static_cast<const C *>(a.mul)[off]);
                 x.y *= sg;
             }
         }
@@ -546,6 +558,7 @@ __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256) fft_batched(batch_
         a.herm_load = 0;
         a.mode = kModeC2C;
         a.stream = 0;
+        a.mul = nullptr;
         const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
         a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
         batch_tile<R>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);

@@ -55,6 +55,8 @@ struct pass_args {
     int herm_load; // 1: positions > n/2 are read as conj(x[n - pos]) (c2r Hermitian extension)
     int mode;      // kModeC2C, kModeR2C or kModeC2R (uniform kernel only)
     int stream;    // 1: non-temporal loads and stores of the data (outputs larger than the MALL)
+    const void *mul;  // non-null: every loaded element is first multiplied by the element of this
+                      // array at the same offset (same layout and type as in; plain c2c loads only)
     double scale;  // multiplies every stored value (cast to the pass precision)
 };
 

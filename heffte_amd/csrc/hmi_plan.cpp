@@ -477,7 +477,7 @@ const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
 }
 
 void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
-                       int scaling) {
+                       int scaling, const void *mul) {
     if (batch < 1) throw std::invalid_argument("batch size must be positive");
     if (!in || !out) throw std::invalid_argument("null input or output array");
     const bool in_place = (in == out);
@@ -548,9 +548,16 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
                                     (p.out_real ? (dbl ? 8 : 4) : (dbl ? 16 : 8));
         a.stream = out_bytes > kStreamBytes ? 1 : 0;
         a.scale = p.last ? scale : 1.0;
+        a.mul = nullptr;
+        if (mul && &p == &ep.passes.front()) {
+            // the fused product needs the plain complex load of the first pass
+            if (kind_ != kind::c2c || p.in_real || p.herm_load || p.mode != kModeC2C || p.src != buf::in)
+                throw std::runtime_error("internal error: the product cannot be fused into this pass");
+            a.mul = mul;
+        }
         const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
         const bool mfma = precision_ != kPrecPlain && !dbl && p.n == 512 && p.mode == kModeC2C && !p.identity &&
-                          !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n;
+                          !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n && !a.mul;
         if (mfma) {
             // the split-precision form: W512 and the split W32 / W16 operand tables, 16-line
             // tiles; the backward pass multiplies by the conjugate matrices directly (no
@@ -575,6 +582,34 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
             const size_t lds = 2 * (size_t)p.gen.lines * p.n * (dbl ? 16 : 8);
             check_hip(launch_generic(dbl, a, p.gen, nblocks, lds, stream_), "generic FFT pass");
         }
+    }
+    if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
+}
+
+void fft_plan::convolve(bool dbl, int batch, const void *x, const void *y, void *out, void *ws, int scaling) {
+    if (kind_ != kind::c2c) throw std::invalid_argument("convolution needs a c2c plan");
+    if (batch < 1) throw std::invalid_argument("batch size must be positive");
+    if (!x || !y || !out) throw std::invalid_argument("null input or output array");
+    const size_t elem = dbl ? 16 : 8;
+    void *tmp = nullptr;
+    if (!ws) {
+        check_hip(hipMallocAsync(&tmp, (size_t)convolve_workspace() * batch * elem, stream_), "hipMallocAsync(workspace)");
+        ws = tmp;
+    }
+    // layout of ws: the transforms' own workspace, then forward(y) in the outbox layout
+    void *yhat = static_cast<char *>(ws) + (size_t)ws_elems_ * batch * elem;
+    try {
+        execute(-1, dbl, false, false, batch, y, yhat, ws, 0);
+        if (x == y) {
+            // one spectrum: the first backward pass squares it on load
+            execute(+1, dbl, false, false, batch, yhat, out, ws, scaling, yhat);
+        } else {
+            execute(-1, dbl, false, false, batch, x, out, ws, 0);
+            execute(+1, dbl, false, false, batch, out, out, ws, scaling, yhat);
+        }
+    } catch (...) {
+        if (tmp) (void)hipFreeAsync(tmp, stream_);
+        throw;
     }
     if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
 }

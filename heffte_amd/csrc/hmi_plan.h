@@ -94,9 +94,16 @@ class fft_plan {
     int precision() const { return precision_; }
 
     // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
-    // overloads of fft3d (real input on forward, real output on backward).
+    // overloads of fft3d (real input on forward, real output on backward).  mul (c2c only):
+    // the first pass loads in[i] * mul[i] instead of in[i] (mul laid out like in).
     void execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
-                 int scaling);
+                 int scaling, const void *mul = nullptr);
+    // out = backward(forward(x) * forward(y)) with the backward scaling (scale::full gives the
+    // circular convolution of x and y); the product is fused into the first backward pass,
+    // and x == y runs one forward transform.
+    // c2c plans only; ws holds batch * convolve_workspace() elements or is null (allocated).
+    void convolve(bool dbl, int batch, const void *x, const void *y, void *out, void *ws, int scaling);
+    long long convolve_workspace() const { return ws_elems_ + outbox_.count(); }
 
     // Plan (without running it): exposed for tests of the planner.
     const exec_plan &get_exec_plan(int dir, bool dbl, bool in_real, bool out_real, bool in_place);

@@ -136,6 +136,22 @@ int heffte_mi355_backward_c2s(heffte_mi355_plan plan, int batch, void const *inp
 int heffte_mi355_backward_z2d(heffte_mi355_plan plan, int batch, void const *input, double *output, void *workspace,
                               int scale);
 
+/* Circular convolution through the transform (no heFFTe entry point: the sequence of
+ * benchmarks/convolution.cpp:86-95 -- forward, pointwise product, backward -- in one call):
+ *   output = backward(forward(x) * forward(y))
+ * on a c2c plan, with the backward scale applied (HMI_SCALE_FULL gives the circular
+ * convolution of x and y; HMI_SCALE_NONE gives N times it).  x, y and output are in the
+ * inbox layout; output may alias x or y.  The product is folded into the first backward
+ * pass, so the product is never written back; x == y (one array) runs one forward
+ * transform and squares its spectrum on load.  workspace holds
+ * batch * heffte_mi355_size_convolve_workspace(plan) complex elements or is NULL (allocated
+ * for the call). */
+long long heffte_mi355_size_convolve_workspace(heffte_mi355_plan plan);
+int heffte_mi355_convolve_c2c(heffte_mi355_plan plan, int batch, void const *x, void const *y, void *output,
+                              void *workspace, int scale);
+int heffte_mi355_convolve_z2z(heffte_mi355_plan plan, int batch, void const *x, void const *y, void *output,
+                              void *workspace, int scale);
+
 /* Real-to-real transforms of an HMI_KIND_COS/SIN/COS1 plan; replace
  * rtransform::forward/backward(float/double data, workspace)
  * (include/heffte_fft3d.h:374-384,480-490 with a real scalar type). */

@@ -121,6 +121,15 @@ def fft3d_c2c(inbox, outbox, x, sign=-1, scale=1.0, batch=1):
     return y
 
 
+def convolve3d(inbox, outbox, x, y, scale=1.0, batch=1):
+    """backward(forward(x) * forward(y)) * scale, x and y in the inbox layout: the sequence of
+    benchmarks/convolution.cpp:86-95 (forward, pointwise product, backward) with the scale
+    on the backward transform; scale = 1/N gives the circular convolution of x and y."""
+    xh = fft3d_c2c(inbox, outbox, x, -1, 1.0, batch)
+    yh = fft3d_c2c(inbox, outbox, y, -1, 1.0, batch)
+    return fft3d_c2c(outbox, inbox, xh * yh, +1, scale, batch)
+
+
 def fft3d_real_c2c(inbox, outbox, x, scale=1.0, batch=1):
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.empty(batch * outbox.count(), dtype=np.complex128)

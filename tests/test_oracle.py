@@ -294,3 +294,51 @@ def test_planted_mistake_is_caught(oracle):
     good = oracle.fft1d(x)
     assert oracle.rel_l2(oracle.fft1d(x, +1), oracle.dft1d_ld(x)) > 0.1
     assert oracle.rel_l2(good, oracle.dft1d_ld(x)) < 1e-14
+
+
+def _circular_convolution(a, b):
+    """Brute force: c[n] = sum_m a[m] b[(n - m) mod N] over the three axes (no transform)."""
+    c = np.zeros(a.shape, dtype=np.complex128)
+    for m in np.ndindex(a.shape):
+        c += a[m] * np.roll(b, m, axis=(0, 1, 2))
+    return c
+
+
+@pytest.mark.parametrize("size,oi,oo", [((4, 3, 5), 0, 0), ((5, 4, 3), 0, 5), ((2, 6, 1), 3, 1)])
+def test_convolve3d_vs_brute_force(oracle, size, oi, oo):
+    """backward(forward(x) * forward(y)) / N is the circular convolution; with scale 1 it is
+    N times it; the product is commutative and a delta at the origin is the identity."""
+    rng = np.random.default_rng(13)
+    inorder, outorder = ORDERS[oi], ORDERS[oo]
+    a = rng.standard_normal(size) + 1j * rng.standard_normal(size)
+    b = rng.standard_normal(size) + 1j * rng.standard_normal(size)
+    bi = box(oracle, (0, 0, 0), [s - 1 for s in size], inorder)
+    bo = box(oracle, (0, 0, 0), [s - 1 for s in size], outorder)
+    ref = _circular_convolution(a, b)
+    c = oracle.convolve3d(bi, bo, flat(a, inorder), flat(b, inorder), 1.0 / a.size)
+    assert oracle.rel_l2(c, flat(ref, inorder)) < TOL64
+    c1 = oracle.convolve3d(bi, bo, flat(b, inorder), flat(a, inorder), 1.0)
+    assert oracle.rel_l2(c1, flat(ref * a.size, inorder)) < TOL64
+    delta = np.zeros(size, dtype=np.complex128)
+    delta[0, 0, 0] = 1.0
+    d = oracle.convolve3d(bi, bo, flat(a, inorder), flat(delta, inorder), 1.0 / a.size)
+    assert oracle.rel_l2(d, flat(a, inorder)) < TOL64
+    # a shifted delta shifts the input (a wrong sign in either transform would shift the other way)
+    delta = np.zeros(size, dtype=np.complex128)
+    delta[1 % size[0], 1 % size[1], 0] = 1.0
+    d = oracle.convolve3d(bi, bo, flat(a, inorder), flat(delta, inorder), 1.0 / a.size)
+    assert oracle.rel_l2(d, flat(np.roll(a, (1, 1), axis=(0, 1)), inorder)) < TOL64
+
+
+def test_convolve3d_batch(oracle):
+    rng = np.random.default_rng(17)
+    size, batch = (3, 4, 2), 3
+    bi = box(oracle, (0, 0, 0), [s - 1 for s in size])
+    bo = box(oracle, (0, 0, 0), [s - 1 for s in size], (2, 0, 1))
+    xs = [rng.standard_normal(size) + 1j * rng.standard_normal(size) for _ in range(batch)]
+    ys = [rng.standard_normal(size) + 1j * rng.standard_normal(size) for _ in range(batch)]
+    x = np.concatenate([flat(a, (0, 1, 2)) for a in xs])
+    y = np.concatenate([flat(a, (0, 1, 2)) for a in ys])
+    c = oracle.convolve3d(bi, bo, x, y, 1.0 / 24, batch)
+    ref = np.concatenate([flat(_circular_convolution(a, b), (0, 1, 2)) for a, b in zip(xs, ys)])
+    assert oracle.rel_l2(c, ref) < TOL64

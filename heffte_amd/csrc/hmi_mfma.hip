@@ -26,6 +26,10 @@
 #include "hmi_device.h"
 #include "hmi_kernels.h"
 
+#ifndef MFMA_TL_CONTIG
+#define MFMA_TL_CONTIG 8
+#endif
+
 namespace hmi {
 
 namespace {
@@ -95,6 +99,23 @@ __device__ __forceinline__ void split8_f16_scaled(const float (&x)[8], float s, 
     __builtin_memcpy(&out[1], lo, sizeof(lo));
 }
 
+// max(m, |a|, |b|) in one instruction
+__device__ __forceinline__ float absmax3(float m, float a, float b) {
+    float r;
+    asm("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
+}
+
+// Maximum of a non-negative value over lanes l, l ^ 16, l ^ 32 and l ^ 48 (the four K groups
+// of a GEMM column) with the gfx950 row swaps; non-negative floats order like their bits.
+__device__ __forceinline__ float max_over_groups(float m) {
+    const unsigned u = __float_as_uint(m);
+    const auto r = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const unsigned v = max(r[0], r[1]);
+    const auto t = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return __uint_as_float(max(t[0], t[1]));
+}
+
 // acc += A * B through the split products
 template <int PREC>
 __device__ __forceinline__ f32x4_t split_mma(const typename split_traits<PREC>::vec (&a)[3],
@@ -124,16 +145,16 @@ __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bo
                : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
 }
 
-// Offsets of line tile * 16 of the input and output arrays when the column count is a
-// multiple of 16 (AL16): the tile's lines then share (b1, batch), and line l of the tile
+// Offsets of line tile * TL of the input and output arrays when the column count is a
+// multiple of TL (AL16): the tile's lines then share (b1, batch), and line l of the tile
 // lies l column strides further.  The divisions run once per tile on wave-uniform values.
 struct tile_pos {
     long long in, out;
 };
-template <bool AL16> __device__ __forceinline__ tile_pos locate_tile(unsigned tile, const pass_args &a) {
+template <bool AL16, int TL> __device__ __forceinline__ tile_pos locate_tile(unsigned tile, const pass_args &a) {
     tile_pos t{0, 0};
     if constexpr (AL16) {
-        const unsigned L0 = tile * 16;
+        const unsigned L0 = tile * TL;
         const unsigned q = L0 / (unsigned)a.ncol;
         const unsigned c0 = L0 - q * (unsigned)a.ncol;
         const unsigned b2 = q / (unsigned)a.nb1;
@@ -143,11 +164,11 @@ template <bool AL16> __device__ __forceinline__ tile_pos locate_tile(unsigned ti
     }
     return t;
 }
-template <bool AL16>
+template <bool AL16, int TL>
 __device__ __forceinline__ long long tile_line_off(const tile_pos &t, unsigned tile, int l, const pass_args &a,
                                                    bool out) {
     if constexpr (AL16) return (out ? t.out : t.in) + (long long)l * (out ? a.out_sc : a.in_sc);
-    else return line_off(tile * 16 + l, a, out);
+    else return line_off(tile * TL + l, a, out);
 }
 
 }  // namespace
@@ -155,11 +176,12 @@ __device__ __forceinline__ long long tile_line_off(const tile_pos &t, unsigned t
 // One tile = 16 lines x 512 points for 1024 threads (16 waves); lane = j + 16 q (j < 16, q < 4).
 //
 // Step A runs straight from the registers the tile was loaded into: thread (wave w, j, q)
-// loads x[16 n1 + n2] for n1 = 8 q + e (e < 8) of one column (n2, line) -- n2 = j, line = w
-// when the axis is contiguous in memory, n2 = w, line = j when the column axis is (16 lines
-// per 128-byte chunk) -- which is exactly its B fragment of v_mfma_f32_16x16x32 (B[k][j] in
-// lane j + 16 (k / 8), element k % 8, k = n1).  Each column is scaled by its own power of
-// two (maximum over the four lanes j, j + 16, j + 32, j + 48: two shuffles), split once,
+// loads x[16 n1 + n2] for n1 = q + 4 e (e < 8) of one column (n2, line) -- n2 = j, line = w
+// when the axis is contiguous in memory (each load then reads 512 contiguous bytes), n2 = w,
+// line = j when the column axis is (16 lines per 128-byte chunk) -- which is its B fragment
+// of v_mfma_f32_16x16x32 (B[k][j] in lane j + 16 (k / 8), element k % 8) with the K slot
+// (q, e) standing for n1 = q + 4 e; the A operands use the same assignment.  Each column is scaled by its own power of
+// two (maximum over the four lanes j, j + 16, j + 32, j + 48: two row swaps), split once,
 // and multiplied by the four 16 x 32 blocks Re F32 / Im F32 of output rows k1 < 16 and
 // k1 >= 16 (real rows: Re F . re - Im F . im, the second product subtracted in FP32;
 // imaginary rows: Im F . re + Re F . im).  The unscaled Y[k1][n2] goes to LDS (line, k1, n2);
@@ -169,18 +191,20 @@ __device__ __forceinline__ long long tile_line_off(const tile_pos &t, unsigned t
 // n2 = 4 q + e / 2), so each lane reads 4 consecutive n2 of Y and of the W512 table, forms
 // the 4 twiddled complex values, scales the column by its own power of two, splits it and
 // runs the two 16 x 32 row blocks of [Re F16, -Im F16; Im F16, Re F16] (columns permuted
-// alike).  Lane (j, q) then holds X[k1 + 32 k2] for k2 = 4 q + r (r < 4): 128-byte chunks
-// on store.
+// alike; row i of a block stands for k2 = 4 (i % 4) + i / 4).  Lane (j, q) then holds
+// X[k1 + 32 k2] for k2 = 4 r + q (r < 4): per store, 128-byte chunks within 1 KiB.
 //
 // Persistent form: one block per CU walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...; the
 // next tile's loads are in flight while the current one computes (two register sets), the Y
 // tile is double-buffered in LDS (one barrier per tile).  a.tw = W512^{n2 k1} as [k1][n2],
 // then the step-A operands [rb][Re/Im][term][lane][8], then step C [row block][term][lane][8].
 // LCF / SCF: the load / store side walks the column axis; AL16: the column count is a
-// multiple of 16 (see tile_line_off).
-template <int PREC, bool LCF, bool SCF, bool AL16>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+// multiple of TL (see tile_line_off); TL: lines per tile = waves per block (16; 8 when both
+// sides are contiguous along the axis, two blocks per CU).
+template <int PREC, bool LCF, bool SCF, bool AL16, int TL>
+__global__ void __launch_bounds__(64 * TL) __attribute__((amdgpu_waves_per_eu(4)))
 fft512_mfma_persistent(pass_args a, unsigned ntiles) {
+    static_assert(TL == 16 || (TL == 8 && !LCF && !SCF), "8-line tiles need the axis contiguous on both sides");
     using S = split_traits<PREC>;
     using vec = typename S::vec;
     constexpr int TERMS = S::terms;
@@ -189,11 +213,17 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     constexpr bool kOpARegs = (TERMS == 2);
     // Y pitches in complex elements: rows of 16 n2 padded to 18 (16-byte aligned, rows of
     // consecutive lanes on distinct banks), lines of 32 rows padded so that lines on
-    // consecutive lanes start on distinct banks
-    constexpr int RP = 18, LP = 32 * RP + 18, TWP = 18;
-    constexpr size_t kYBytes = sizeof(float2) * 16 * LP;
-    constexpr size_t kTw = 2 * kYBytes, kOpC = kTw + sizeof(float2) * 32 * TWP;
-    constexpr size_t kBytes = kOpC + sizeof(vec) * 2 * TERMS * 64;
+    // consecutive lanes start on distinct banks (only when lanes walk lines)
+    constexpr int RP = 18, LP = 32 * RP + ((LCF || SCF) ? 18 : 0), TWP = 18;
+    constexpr size_t kYBytes = sizeof(float2) * TL * LP;
+    // FP16 keeps each lane's 8 step-C twiddles in registers; BF16 (no registers left) reads
+    // them from an LDS copy of the table
+    constexpr bool kTwRegs = (TERMS == 2);
+    constexpr size_t kTw = 2 * kYBytes, kOpC = kTw + (kTwRegs ? 0 : sizeof(float2) * 32 * TWP);
+    // the step-C operands live in LDS for 16-line tiles; two 8-line blocks per CU leave no
+    // room for them and read the cached table instead
+    constexpr bool kOpCLds = (TL == 16);
+    constexpr size_t kBytes = kOpC + (kOpCLds ? sizeof(vec) * 2 * TERMS * 64 : 0);
     __shared__ __align__(16) unsigned char lds[kBytes];
     float2 *tws = reinterpret_cast<float2 *>(lds + kTw);
     vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
@@ -204,8 +234,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     const vec *__restrict__ opC = opA + 2 * 2 * TERMS * 64;
     {
         const int tid = threadIdx.x;
-        if (tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
-        if (tid < 512) tws[(tid >> 4) * TWP + (tid & 15)] = w512[tid];
+        if (kOpCLds && tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
+        if (!kTwRegs && tid < 512) tws[(tid >> 4) * TWP + (tid & 15)] = w512[tid];
     }
     const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
 
@@ -222,10 +252,10 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         const int tid = thread();
         const int w = __builtin_amdgcn_readfirstlane(tid >> 6), j = tid & 15, q = (tid >> 4) & 3;
         const int line = LCF ? j : w, n2 = LCF ? w : j;
-        const unsigned L = tile * 16 + line;
-        const float2 *p = src + (L < nlines ? tile_line_off<AL16>(tp, tile, line, a, false) : 0) +
-                          (long long)(128 * q + n2) * a.in_sa;
-        const long long step = 16 * a.in_sa;
+        const unsigned L = tile * TL + line;
+        const float2 *p = src + (L < nlines ? tile_line_off<AL16, TL>(tp, tile, line, a, false) : 0) +
+                          (long long)(16 * q + n2) * a.in_sa;
+        const long long step = 64 * a.in_sa;
 #pragma unroll
         for (int e = 0; e < 8; e++) x[e] = ld_stream(&p[e * step]);
     };
@@ -241,6 +271,18 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                 for (int tt = 0; tt < TERMS; tt++) ra[rb][mt][tt] = opA[((rb * 2 + mt) * TERMS + tt) * 64 + lane];
     }
 
+    float2 twr[2][4];
+    if constexpr (kTwRegs) {
+        const int tid = threadIdx.x;
+        const int w = tid >> 6, j = tid & 15, q = (tid >> 4) & 3;
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int k1 = SCF ? 2 * w + cb : 16 * cb + j;
+#pragma unroll
+            for (int p = 0; p < 4; p++) twr[cb][p] = w512[k1 * 16 + 4 * q + p];
+        }
+    }
+
     auto run_tile = [&](unsigned tile, const tile_pos &tp, int par, const float2 (&x)[8]) {
         float2 *ybuf = reinterpret_cast<float2 *>(lds + par * kYBytes);
         // ---- column scale, split, step A
@@ -251,9 +293,8 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
             const int line = LCF ? j : w, n2 = LCF ? w : j;
             float m = 0.f;
 #pragma unroll
-            for (int e = 0; e < 8; e++) m = fmaxf(m, fmaxf(fabsf(x[e].x), fabsf(x[e].y)));
-            m = fmaxf(m, __shfl_xor(m, 16));
-            m = fmaxf(m, __shfl_xor(m, 32));
+            for (int e = 0; e < 8; e++) m = absmax3(m, x[e].x, x[e].y);
+            m = max_over_groups(m);
             int ex = 0;
             (void)frexpf(m, &ex);
             const float s = ldexpf(1.0f, -ex);
@@ -312,14 +353,14 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
 #pragma unroll
             for (int rc = 0; rc < 2; rc++)
 #pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) fc[rc][tt] = opCs[(rc * TERMS + tt) * 64 + lane];
+                for (int tt = 0; tt < TERMS; tt++) fc[rc][tt] = (kOpCLds ? opCs : opC)[(rc * TERMS + tt) * 64 + lane];
 #pragma unroll
             for (int cb = 0; cb < 2; cb++) {
                 const int line = SCF ? j : w, k1 = SCF ? 2 * w + cb : 16 * cb + j;
                 // lane (j, q): z = W512^{n2 k1} Y[k1][n2] for n2 = 4 q + p, p < 4, as
                 // (re, im) pairs: element e of the B fragment is part e & 1 of n2 = 4 q + e / 2
                 const float2 *yp = &ybuf[line * LP + k1 * RP + 4 * q];
-                const float2 *wp = &tws[k1 * TWP + 4 * q];
+                const float2 *wp = kTwRegs ? twr[cb] : &tws[k1 * TWP + 4 * q];
                 float z[8];
                 float m = 0.f;
 #pragma unroll
@@ -327,10 +368,9 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                     const float2 y = yp[p], t = wp[p];
                     z[2 * p] = y.x * t.x - y.y * t.y;
                     z[2 * p + 1] = y.x * t.y + y.y * t.x;
-                    m = fmaxf(m, fmaxf(fabsf(z[2 * p]), fabsf(z[2 * p + 1])));
+                    m = absmax3(m, z[2 * p], z[2 * p + 1]);
                 }
-                m = fmaxf(m, __shfl_xor(m, 16));
-                m = fmaxf(m, __shfl_xor(m, 32));
+                m = max_over_groups(m);
                 int ex = 0;
                 (void)frexpf(m, &ex);
                 const float s = ldexpf(1.0f, -ex);
@@ -344,13 +384,13 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
                 }
                 const f32x4_t xr = split_mma<PREC>(fc[0], b, f32x4_t{});
                 const f32x4_t xi = split_mma<PREC>(fc[1], b, f32x4_t{});
-                const unsigned L = tile * 16 + line;
+                const unsigned L = tile * TL + line;
                 if (L < nlines) {
                     const float sc = ldexpf((float)a.scale, ex);
                     float2 *__restrict__ p = static_cast<float2 *>(a.out) +
-                                             tile_line_off<AL16>(tp, tile, line, a, true) +
-                                             (long long)(k1 + 128 * q) * a.out_sa;
-                    const long long step = 32 * a.out_sa;
+                                             tile_line_off<AL16, TL>(tp, tile, line, a, true) +
+                                             (long long)(k1 + 32 * q) * a.out_sa;
+                    const long long step = 128 * a.out_sa;
 #pragma unroll
                     for (int r = 0; r < 4; r++) st_stream(&p[r * step], make_float2(xr[r] * sc, xi[r] * sc));
                 }
@@ -361,13 +401,13 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
     // two register sets: the loads of tile t + gridDim.x go to the set tile t does not use
     unsigned tile = blockIdx.x;
     float2 xa[8], xb[8];
-    tile_pos pa = locate_tile<AL16>(tile, a), pb{0, 0};
+    tile_pos pa = locate_tile<AL16, TL>(tile, a), pb{0, 0};
     issue(tile, pa, xa);
     int par = 0;
     for (;;) {
         unsigned next = tile + gridDim.x;
         if (next < ntiles) {
-            pb = locate_tile<AL16>(next, a);
+            pb = locate_tile<AL16, TL>(next, a);
             issue(next, pb, xb);
         }
         run_tile(tile, pa, par, xa);
@@ -376,7 +416,7 @@ fft512_mfma_persistent(pass_args a, unsigned ntiles) {
         if (tile >= ntiles) break;
         next = tile + gridDim.x;
         if (next < ntiles) {
-            pa = locate_tile<AL16>(next, a);
+            pa = locate_tile<AL16, TL>(next, a);
             issue(next, pa, xa);
         }
         run_tile(tile, pb, par, xb);
@@ -404,25 +444,26 @@ template <typename S> void split_store(float v, int terms, S *out, int stride) {
 }
 template <typename S> void fill_operands(std::vector<unsigned char> &bytes, int terms, bool conj) {
     // step A: row block rb (k1 = 16 rb + i), matrix mt (0: Re F32, 1: Im F32), lane i + 16 q,
-    // element e: K index n1 = 8 q + e
+    // element e: K index n1 = q + 4 e
     const size_t nA = 2 * 2 * terms * 64 * 8, nC = 2 * terms * 64 * 8;
     std::vector<S> A(nA), Cm(nC);
     for (int rb = 0; rb < 2; rb++)
         for (int mt = 0; mt < 2; mt++)
             for (int lane = 0; lane < 64; lane++)
                 for (int e = 0; e < 8; e++) {
-                    const int k1 = 16 * rb + (lane & 15), n1 = 8 * (lane >> 4) + e;
+                    const int k1 = 16 * rb + (lane & 15), n1 = (lane >> 4) + 4 * e;
                     long double wr, wi;
                     wroot((long long)k1 * n1, 32, wr, wi, conj);
                     split_store<S>((float)(mt == 0 ? wr : wi), terms,
                                    &A[(((size_t)(rb * 2 + mt) * terms) * 64 + lane) * 8 + e], 64 * 8);
                 }
-    // step C: row block rc = 0 gives Re X[k1 + 32 k2], rc = 1 Im X (k2 = lane & 15); element e
-    // of lane group q multiplies part e & 1 (0: real, 1: imaginary) of z[n2 = 4 q + e / 2]
+    // step C: row block rc = 0 gives Re X[k1 + 32 k2], rc = 1 Im X, row i = lane & 15 holding
+    // k2 = 4 (i % 4) + i / 4; element e of lane group q multiplies part e & 1 (0: real,
+    // 1: imaginary) of z[n2 = 4 q + e / 2]
     for (int rc = 0; rc < 2; rc++)
         for (int lane = 0; lane < 64; lane++)
             for (int e = 0; e < 8; e++) {
-                const int k2 = lane & 15, n2 = 4 * (lane >> 4) + (e >> 1), part = e & 1;
+                const int i = lane & 15, k2 = 4 * (i & 3) + (i >> 2), n2 = 4 * (lane >> 4) + (e >> 1), part = e & 1;
                 long double wr, wi;
                 wroot((long long)k2 * n2, 16, wr, wi, conj);
                 const long double v = (rc == 0) ? (part == 0 ? wr : -wi) : (part == 0 ? wi : wr);
@@ -452,20 +493,23 @@ std::vector<unsigned char> mfma512_operand_table(int prec, bool conj) {
 }
 
 namespace {
-template <int PREC, bool LCF, bool SCF>
-void launch_mfma512_al(const pass_args &a, long long grid, long long ntiles, hipStream_t s) {
-    if ((a.ncol & 15) == 0)
-        fft512_mfma_persistent<PREC, LCF, SCF, true><<<dim3((unsigned)grid), dim3(1024), 0, s>>>(a, (unsigned)ntiles);
+template <int PREC, bool LCF, bool SCF, int TL>
+void launch_mfma512_al(const pass_args &a, int cus, hipStream_t s) {
+    const long long ntiles = ((long long)a.nlines + TL - 1) / TL;
+    // persistent: one 16-line block or two 8-line blocks per CU, never more blocks than tiles
+    const long long grid = std::min<long long>(ntiles, (long long)std::max(1, cus) * (16 / TL));
+    if ((a.ncol % TL) == 0)
+        fft512_mfma_persistent<PREC, LCF, SCF, true, TL><<<dim3((unsigned)grid), dim3(64 * TL), 0, s>>>(a, (unsigned)ntiles);
     else
-        fft512_mfma_persistent<PREC, LCF, SCF, false><<<dim3((unsigned)grid), dim3(1024), 0, s>>>(a, (unsigned)ntiles);
+        fft512_mfma_persistent<PREC, LCF, SCF, false, TL><<<dim3((unsigned)grid), dim3(64 * TL), 0, s>>>(a, (unsigned)ntiles);
 }
-template <int PREC> void launch_mfma512_prec(const pass_args &a, long long grid, long long ntiles, hipStream_t s) {
+template <int PREC> void launch_mfma512_prec(const pass_args &a, int cus, hipStream_t s) {
     if (a.load_colfast) {
-        if (a.store_colfast) launch_mfma512_al<PREC, true, true>(a, grid, ntiles, s);
-        else launch_mfma512_al<PREC, true, false>(a, grid, ntiles, s);
+        if (a.store_colfast) launch_mfma512_al<PREC, true, true, 16>(a, cus, s);
+        else launch_mfma512_al<PREC, true, false, 16>(a, cus, s);
     } else {
-        if (a.store_colfast) launch_mfma512_al<PREC, false, true>(a, grid, ntiles, s);
-        else launch_mfma512_al<PREC, false, false>(a, grid, ntiles, s);
+        if (a.store_colfast) launch_mfma512_al<PREC, false, true, 16>(a, cus, s);
+        else launch_mfma512_al<PREC, false, false, MFMA_TL_CONTIG>(a, cus, s);
     }
 }
 }  // namespace
@@ -473,14 +517,12 @@ template <int PREC> void launch_mfma512_prec(const pass_args &a, long long grid,
 hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
     if (prec != kSplitF16 && prec != kSplitBF16) return hipErrorInvalidValue;
-    // one block per CU, never more blocks than tiles
     int dev = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    const long long grid = std::min<long long>(nblocks, std::max(1, cus));
-    if (prec == kSplitF16) launch_mfma512_prec<kSplitF16>(a, grid, nblocks, s);
-    else launch_mfma512_prec<kSplitBF16>(a, grid, nblocks, s);
+    if (prec == kSplitF16) launch_mfma512_prec<kSplitF16>(a, cus, s);
+    else launch_mfma512_prec<kSplitBF16>(a, cus, s);
     return hipGetLastError();
 }
 

@@ -48,8 +48,8 @@ hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds
 // Precision forms of the c2c passes: plain FP32/FP64 vector ALU, or the 512-point
 // complex<float> pass on the matrix cores with split-precision operands.
 enum : int { kPrecPlain = 0, kSplitF16 = 1, kSplitBF16 = 2 };
-// 512-point complex<float> c2c pass on MFMA (nblocks = number of 16-line tiles, walked by one
-// persistent block per CU); a.tw = the bytes of
+// 512-point complex<float> c2c pass on MFMA (nblocks = number of 16-line tiles, one
+// 1024-thread block each); a.tw = the bytes of
 // mfma512_operand_table(prec, conj): the W512 table and the split DFT-matrix operands of the
 // forward transform, or their conjugates for the backward one (the kernel conjugates nothing).
 hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s);

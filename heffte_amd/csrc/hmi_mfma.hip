@@ -26,9 +26,6 @@
 #include "hmi_device.h"
 #include "hmi_kernels.h"
 
-#ifndef MFMA_TL_CONTIG
-#define MFMA_TL_CONTIG 8
-#endif
 
 namespace hmi {
 
@@ -145,16 +142,16 @@ __device__ __forceinline__ long long line_off(unsigned L, const pass_args &a, bo
                : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
 }
 
-// Offsets of line tile * TL of the input and output arrays when the column count is a
-// multiple of TL (AL16): the tile's lines then share (b1, batch), and line l of the tile
+// Offsets of line tile * 16 of the input and output arrays when the column count is a
+// multiple of 16 (AL16): the tile's lines then share (b1, batch), and line l of the tile
 // lies l column strides further.  The divisions run once per tile on wave-uniform values.
 struct tile_pos {
     long long in, out;
 };
-template <bool AL16, int TL> __device__ __forceinline__ tile_pos locate_tile(unsigned tile, const pass_args &a) {
+template <bool AL16> __device__ __forceinline__ tile_pos locate_tile(unsigned tile, const pass_args &a) {
     tile_pos t{0, 0};
     if constexpr (AL16) {
-        const unsigned L0 = tile * TL;
+        const unsigned L0 = tile * 16;
         const unsigned q = L0 / (unsigned)a.ncol;
         const unsigned c0 = L0 - q * (unsigned)a.ncol;
         const unsigned b2 = q / (unsigned)a.nb1;
@@ -164,265 +161,214 @@ template <bool AL16, int TL> __device__ __forceinline__ tile_pos locate_tile(uns
     }
     return t;
 }
-template <bool AL16, int TL>
+template <bool AL16>
 __device__ __forceinline__ long long tile_line_off(const tile_pos &t, unsigned tile, int l, const pass_args &a,
                                                    bool out) {
     if constexpr (AL16) return (out ? t.out : t.in) + (long long)l * (out ? a.out_sc : a.in_sc);
-    else return line_off(tile * TL + l, a, out);
+    else return line_off(tile * 16 + l, a, out);
 }
+
+// Y tile in LDS, unpadded, with a swizzle that spreads the accesses of a wave over the banks:
+// element (line, k1, n2) at line * 512 + pr * 16 + (n2 ^ sw), pr = k1 with its low bit
+// flipped by bit 2 of k1 and bit 3 of the line, sw = 2 ((k1 / 2 % 8) ^ (line % 8)) (even, so
+// the pairs (2m, 2m + 1) read as one 16-byte access stay in order)
+__device__ __forceinline__ int yswz(int line, int k1, int n2) {
+    const int pr = k1 ^ (((k1 >> 2) ^ (line >> 3)) & 1);
+    const int sw = (((k1 >> 1) & 7) ^ (line & 7)) << 1;
+    return line * 512 + pr * 16 + (n2 ^ sw);
+}
+// W512 table [k1][n2] with the row swizzle of yswz for line 0
+__device__ __forceinline__ int twswz(int k1, int n2) { return k1 * 16 + (n2 ^ (((k1 >> 1) & 7) << 1)); }
 
 }  // namespace
 
-// One tile = 16 lines x 512 points for 1024 threads (16 waves); lane = j + 16 q (j < 16, q < 4).
+// One tile = 16 lines x 512 points for one 1024-thread block (16 waves); lane = j + 16 q
+// (j < 16, q < 4).
 //
 // Step A runs straight from the registers the tile was loaded into: thread (wave w, j, q)
 // loads x[16 n1 + n2] for n1 = q + 4 e (e < 8) of one column (n2, line) -- n2 = j, line = w
 // when the axis is contiguous in memory (each load then reads 512 contiguous bytes), n2 = w,
 // line = j when the column axis is (16 lines per 128-byte chunk) -- which is its B fragment
 // of v_mfma_f32_16x16x32 (B[k][j] in lane j + 16 (k / 8), element k % 8) with the K slot
-// (q, e) standing for n1 = q + 4 e; the A operands use the same assignment.  Each column is scaled by its own power of
-// two (maximum over the four lanes j, j + 16, j + 32, j + 48: two row swaps), split once,
-// and multiplied by the four 16 x 32 blocks Re F32 / Im F32 of output rows k1 < 16 and
-// k1 >= 16 (real rows: Re F . re - Im F . im, the second product subtracted in FP32;
-// imaginary rows: Im F . re + Re F . im).  The unscaled Y[k1][n2] goes to LDS (line, k1, n2);
-// after one barrier each wave takes 32 columns (k1, line) of step C -- k1 = 16 cb + j of
-// line w, or k1 = 2 w + cb of line j when the store walks the column axis.  Its K index
-// interleaves real and imaginary parts (element e of lane group q is part e & 1 of
-// n2 = 4 q + e / 2), so each lane reads 4 consecutive n2 of Y and of the W512 table, forms
-// the 4 twiddled complex values, scales the column by its own power of two, splits it and
-// runs the two 16 x 32 row blocks of [Re F16, -Im F16; Im F16, Re F16] (columns permuted
-// alike; row i of a block stands for k2 = 4 (i % 4) + i / 4).  Lane (j, q) then holds
-// X[k1 + 32 k2] for k2 = 4 r + q (r < 4): per store, 128-byte chunks within 1 KiB.
+// (q, e) standing for n1 = q + 4 e; the A operands use the same assignment.  Each column is
+// scaled by its own power of two (maximum over the four lanes j, j + 16, j + 32, j + 48: two
+// row swaps), split once, and multiplied by the 16 x 32 blocks Re F32 / Im F32 of output rows
+// k1 < 16 and k1 >= 16 (real rows: Re F . re - Im F . im, the second product subtracted in
+// FP32; imaginary rows: Im F . re + Re F . im), one operand matrix live at a time.  The
+// unscaled Y[k1][n2] goes to LDS; after a barrier each wave takes 32 columns (k1, line) of
+// step C -- k1 = 16 cb + j of line w, or k1 = 2 w + cb of line j when the store walks the
+// column axis.  Its K index interleaves real and imaginary parts (element e of lane group q
+// is part e & 1 of n2 = 4 q + e / 2), so each lane reads 4 consecutive n2 of Y and of the
+// W512 table, forms the 4 twiddled complex values, scales the column by its own power of two,
+// splits it and runs the two 16 x 32 row blocks of [Re F16, -Im F16; Im F16, Re F16] (columns
+// permuted alike; row i of a block stands for k2 = 4 (i % 4) + i / 4).  Lane (j, q) then
+// holds X[k1 + 32 k2] for k2 = 4 r + q (r < 4): per store, 128-byte chunks within 1 KiB.
 //
-// Persistent form: one block per CU walks tiles blockIdx.x, blockIdx.x + gridDim.x, ...; the
-// next tile's loads are in flight while the current one computes (two register sets), the Y
-// tile is double-buffered in LDS (one barrier per tile).  a.tw = W512^{n2 k1} as [k1][n2],
-// then the step-A operands [rb][Re/Im][term][lane][8], then step C [row block][term][lane][8].
+// Occupancy: at most 64 VGPRs and 80 KB of LDS, so two blocks (8 waves per SIMD) share a CU
+// and hide each other's memory latency; no tile is prefetched.  a.tw = W512^{n2 k1} as
+// [k1][n2], then the step-A operands [rb][Re/Im][term][lane][8], then step C [row
+// block][term][lane][8]; each block copies the W512 table, the step-C operands and (FP16) the
+// step-A operands to LDS while its loads are in flight; BF16 reads its step-A operands from
+// the cached table (no LDS left for three terms).
 // LCF / SCF: the load / store side walks the column axis; AL16: the column count is a
-// multiple of TL (see tile_line_off); TL: lines per tile = waves per block (16; 8 when both
-// sides are contiguous along the axis, two blocks per CU).
-template <int PREC, bool LCF, bool SCF, bool AL16, int TL>
-__global__ void __launch_bounds__(64 * TL) __attribute__((amdgpu_waves_per_eu(4)))
-fft512_mfma_persistent(pass_args a, unsigned ntiles) {
-    static_assert(TL == 16 || (TL == 8 && !LCF && !SCF), "8-line tiles need the axis contiguous on both sides");
+// multiple of 16 (see tile_line_off).
+template <int PREC, bool LCF, bool SCF, bool AL16>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+fft512_mfma(pass_args a) {
     using S = split_traits<PREC>;
     using vec = typename S::vec;
     constexpr int TERMS = S::terms;
-    // FP16 keeps each lane's step-A operands (32 VGPRs) in registers; BF16 (three terms)
-    // reads them from the cached table every tile
-    constexpr bool kOpARegs = (TERMS == 2);
-    // Y pitches in complex elements: rows of 16 n2 padded to 18 (16-byte aligned, rows of
-    // consecutive lanes on distinct banks), lines of 32 rows padded so that lines on
-    // consecutive lanes start on distinct banks (only when lanes walk lines)
-    constexpr int RP = 18, LP = 32 * RP + ((LCF || SCF) ? 18 : 0), TWP = 18;
-    constexpr size_t kYBytes = sizeof(float2) * TL * LP;
-    // FP16 keeps each lane's 8 step-C twiddles in registers; BF16 (no registers left) reads
-    // them from an LDS copy of the table
-    constexpr bool kTwRegs = (TERMS == 2);
-    constexpr size_t kTw = 2 * kYBytes, kOpC = kTw + (kTwRegs ? 0 : sizeof(float2) * 32 * TWP);
-    // the step-C operands live in LDS for 16-line tiles; two 8-line blocks per CU leave no
-    // room for them and read the cached table instead
-    constexpr bool kOpCLds = (TL == 16);
-    constexpr size_t kBytes = kOpC + (kOpCLds ? sizeof(vec) * 2 * TERMS * 64 : 0);
+    constexpr bool kOpALds = (TERMS == 2);
+    constexpr size_t kY = 0, kTw = kY + sizeof(float2) * 16 * 512, kOpC = kTw + sizeof(float2) * 512;
+    constexpr size_t kOpA = kOpC + sizeof(vec) * 2 * TERMS * 64;
+    constexpr size_t kBytes = kOpA + (kOpALds ? sizeof(vec) * 4 * TERMS * 64 : 0);
     __shared__ __align__(16) unsigned char lds[kBytes];
+    float2 *ybuf = reinterpret_cast<float2 *>(lds + kY);
     float2 *tws = reinterpret_cast<float2 *>(lds + kTw);
     vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
+    vec *opAs = reinterpret_cast<vec *>(lds + kOpA);
 
+    const unsigned tile = blockIdx.x;
     const unsigned nlines = (unsigned)a.nlines;
     const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
     const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);
     const vec *__restrict__ opC = opA + 2 * 2 * TERMS * 64;
-    {
-        const int tid = threadIdx.x;
-        if (kOpCLds && tid < 2 * TERMS * 64) opCs[tid] = opC[tid];
-        if (!kTwRegs && tid < 512) tws[(tid >> 4) * TWP + (tid & 15)] = w512[tid];
-    }
-    const float2 *__restrict__ src = static_cast<const float2 *>(a.in);
+    const tile_pos tp = locate_tile<AL16>(tile, a);
 
-    // The thread index is passed through an empty asm at each use so that the per-lane
-    // addresses derived from it are recomputed where needed instead of being kept live
-    // across the loop (128 VGPRs at this occupancy).
-    auto thread = [] {
-        int v = threadIdx.x;
-        asm volatile("" : "+v"(v));
-        return v;
-    };
-    // the eight loads of a tile (an invalid line reads line 0; its results are never stored)
-    auto issue = [&](unsigned tile, const tile_pos &tp, float2 (&x)[8]) {
-        const int tid = thread();
-        const int w = __builtin_amdgcn_readfirstlane(tid >> 6), j = tid & 15, q = (tid >> 4) & 3;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, j = tid & 15, q = (tid >> 4) & 3;
+    // the tile's loads first, then the operand tables to LDS
+    float2 x[8];
+    {
         const int line = LCF ? j : w, n2 = LCF ? w : j;
-        const unsigned L = tile * TL + line;
-        const float2 *p = src + (L < nlines ? tile_line_off<AL16, TL>(tp, tile, line, a, false) : 0) +
+        const unsigned L = tile * 16 + line;
+        const float2 *p = static_cast<const float2 *>(a.in) +
+                          (L < nlines ? tile_line_off<AL16>(tp, tile, line, a, false) : 0) +
                           (long long)(16 * q + n2) * a.in_sa;
         const long long step = 64 * a.in_sa;
 #pragma unroll
         for (int e = 0; e < 8; e++) x[e] = ld_stream(&p[e * step]);
-    };
-
-    vec ra[2][2][TERMS];  // [rb][Re F, Im F][term]
-    if constexpr (kOpARegs) {
-        const int lane = threadIdx.x & 63;
-#pragma unroll
-        for (int rb = 0; rb < 2; rb++)
-#pragma unroll
-            for (int mt = 0; mt < 2; mt++)
-#pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) ra[rb][mt][tt] = opA[((rb * 2 + mt) * TERMS + tt) * 64 + lane];
     }
+    if (tid < 256) {
+        const int k1 = tid >> 3, n2 = 2 * (tid & 7);
+        const float4 v = reinterpret_cast<const float4 *>(w512)[tid];
+        *reinterpret_cast<float4 *>(&tws[twswz(k1, n2)]) = v;
+    } else if (tid < 256 + 2 * TERMS * 64) {
+        opCs[tid - 256] = opC[tid - 256];
+    } else if (kOpALds && tid < 256 + 6 * TERMS * 64) {
+        opAs[tid - 256 - 2 * TERMS * 64] = opA[tid - 256 - 2 * TERMS * 64];
+    }
+    const vec *opAsrc = kOpALds ? opAs : opA;
 
-    float2 twr[2][4];
-    if constexpr (kTwRegs) {
-        const int tid = threadIdx.x;
-        const int w = tid >> 6, j = tid & 15, q = (tid >> 4) & 3;
+    // column scale and split (before the barrier: no LDS involved)
+    vec bre[3], bim[3];
+    int exA = 0;
+    {
+        float m = 0.f;
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
-            const int k1 = SCF ? 2 * w + cb : 16 * cb + j;
+        for (int e = 0; e < 8; e++) m = absmax3(m, x[e].x, x[e].y);
+        m = max_over_groups(m);
+        (void)frexpf(m, &exA);
+        const float s = ldexpf(1.0f, -exA);
+        float re[8], im[8];
 #pragma unroll
-            for (int p = 0; p < 4; p++) twr[cb][p] = w512[k1 * 16 + 4 * q + p];
+        for (int e = 0; e < 8; e++) {
+            if constexpr (PREC == kSplitF16) {
+                re[e] = x[e].x;
+                im[e] = x[e].y;
+            } else {
+                re[e] = x[e].x * s;
+                im[e] = x[e].y * s;
+            }
+        }
+        if constexpr (PREC == kSplitF16) {
+            split8_f16_scaled(re, s, bre);
+            split8_f16_scaled(im, s, bim);
+        } else {
+            split8<PREC>(re, bre);
+            split8<PREC>(im, bim);
         }
     }
-
-    auto run_tile = [&](unsigned tile, const tile_pos &tp, int par, const float2 (&x)[8]) {
-        float2 *ybuf = reinterpret_cast<float2 *>(lds + par * kYBytes);
-        // ---- column scale, split, step A
-        {
-            const int tid = thread();
-            const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, j = tid & 15,
-                      q = (tid >> 4) & 3;
-            const int line = LCF ? j : w, n2 = LCF ? w : j;
+    __syncthreads();  // operand tables in LDS
+    {
+        const int line = LCF ? j : w, n2 = LCF ? w : j;
+        const float u = ldexpf(1.0f, exA);
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            // one operand matrix live at a time: Re F (real rows . re, imaginary rows . im),
+            // then Im F
+            vec f[3];
+#pragma unroll
+            for (int tt = 0; tt < TERMS; tt++) f[tt] = opAsrc[((rb * 2 + 0) * TERMS + tt) * 64 + lane];
+            const f32x4_t pr = split_mma<PREC>(f, bre, f32x4_t{});
+            f32x4_t yi = split_mma<PREC>(f, bim, f32x4_t{});
+#pragma unroll
+            for (int tt = 0; tt < TERMS; tt++) f[tt] = opAsrc[((rb * 2 + 1) * TERMS + tt) * 64 + lane];
+            const f32x4_t pi = split_mma<PREC>(f, bim, f32x4_t{});
+            yi = split_mma<PREC>(f, bre, yi);
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                ybuf[yswz(line, 16 * rb + 4 * q + r, n2)] = make_float2((pr[r] - pi[r]) * u, yi[r] * u);
+        }
+    }
+    __syncthreads();
+    {
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int line = SCF ? j : w, k1 = SCF ? 2 * w + cb : 16 * cb + j;
+            float2 y[4], t[4];
+            {
+                const float4 y0 = *reinterpret_cast<const float4 *>(&ybuf[yswz(line, k1, 4 * q)]);
+                const float4 y1 = *reinterpret_cast<const float4 *>(&ybuf[yswz(line, k1, 4 * q + 2)]);
+                const float4 t0 = *reinterpret_cast<const float4 *>(&tws[twswz(k1, 4 * q)]);
+                const float4 t1 = *reinterpret_cast<const float4 *>(&tws[twswz(k1, 4 * q + 2)]);
+                y[0] = make_float2(y0.x, y0.y);
+                y[1] = make_float2(y0.z, y0.w);
+                y[2] = make_float2(y1.x, y1.y);
+                y[3] = make_float2(y1.z, y1.w);
+                t[0] = make_float2(t0.x, t0.y);
+                t[1] = make_float2(t0.z, t0.w);
+                t[2] = make_float2(t1.x, t1.y);
+                t[3] = make_float2(t1.z, t1.w);
+            }
+            float z[8];
             float m = 0.f;
 #pragma unroll
-            for (int e = 0; e < 8; e++) m = absmax3(m, x[e].x, x[e].y);
+            for (int p = 0; p < 4; p++) {
+                z[2 * p] = y[p].x * t[p].x - y[p].y * t[p].y;
+                z[2 * p + 1] = y[p].x * t[p].y + y[p].y * t[p].x;
+                m = absmax3(m, z[2 * p], z[2 * p + 1]);
+            }
             m = max_over_groups(m);
             int ex = 0;
             (void)frexpf(m, &ex);
             const float s = ldexpf(1.0f, -ex);
-            float re[8], im[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                if constexpr (PREC == kSplitF16) {
-                    re[e] = x[e].x;
-                    im[e] = x[e].y;
-                } else {
-                    re[e] = x[e].x * s;
-                    im[e] = x[e].y * s;
-                }
-            }
-            vec bre[3], bim[3];
+            vec b[3];
             if constexpr (PREC == kSplitF16) {
-                split8_f16_scaled(re, s, bre);
-                split8_f16_scaled(im, s, bim);
+                split8_f16_scaled(z, s, b);
             } else {
-                split8<PREC>(re, bre);
-                split8<PREC>(im, bim);
+#pragma unroll
+                for (int e = 0; e < 8; e++) z[e] *= s;
+                split8<PREC>(z, b);
             }
-            const float u = ldexpf(1.0f, ex);
-            float2 *yp = &ybuf[line * LP + 4 * q * RP + n2];
+            vec fc[3];
 #pragma unroll
-            for (int rb = 0; rb < 2; rb++) {
-                vec fre[3], fim[3];
+            for (int tt = 0; tt < TERMS; tt++) fc[tt] = opCs[(0 * TERMS + tt) * 64 + lane];
+            const f32x4_t xr = split_mma<PREC>(fc, b, f32x4_t{});
 #pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) {
-                    if constexpr (kOpARegs) {
-                        fre[tt] = ra[rb][0][tt];
-                        fim[tt] = ra[rb][1][tt];
-                    } else {
-                        fre[tt] = opA[((rb * 2 + 0) * TERMS + tt) * 64 + lane];
-                        fim[tt] = opA[((rb * 2 + 1) * TERMS + tt) * 64 + lane];
-                    }
-                }
-                // real rows Re F . re - Im F . im (the second product subtracted in FP32),
-                // imaginary rows Im F . re + Re F . im
-                const f32x4_t pr = split_mma<PREC>(fre, bre, f32x4_t{});
-                const f32x4_t pi = split_mma<PREC>(fim, bim, f32x4_t{});
-                f32x4_t yi = split_mma<PREC>(fim, bre, f32x4_t{});
-                yi = split_mma<PREC>(fre, bim, yi);
-                // lane (j, q) element r: k1 = 16 rb + 4 q + r
+            for (int tt = 0; tt < TERMS; tt++) fc[tt] = opCs[(1 * TERMS + tt) * 64 + lane];
+            const f32x4_t xi = split_mma<PREC>(fc, b, f32x4_t{});
+            const unsigned L = tile * 16 + line;
+            if (L < nlines) {
+                const float sc = ldexpf((float)a.scale, ex);
+                float2 *__restrict__ p = static_cast<float2 *>(a.out) +
+                                         tile_line_off<AL16>(tp, tile, line, a, true) +
+                                         (long long)(k1 + 32 * q) * a.out_sa;
+                const long long step = 128 * a.out_sa;
 #pragma unroll
-                for (int r = 0; r < 4; r++) yp[(16 * rb + r) * RP] = make_float2((pr[r] - pi[r]) * u, yi[r] * u);
+                for (int r = 0; r < 4; r++) st_stream(&p[r * step], make_float2(xr[r] * sc, xi[r] * sc));
             }
         }
-        __syncthreads();
-        // ---- twiddle, column scale, split, step C, store
-        {
-            const int tid = thread();
-            const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, j = tid & 15,
-                      q = (tid >> 4) & 3;
-            vec fc[2][3];
-#pragma unroll
-            for (int rc = 0; rc < 2; rc++)
-#pragma unroll
-                for (int tt = 0; tt < TERMS; tt++) fc[rc][tt] = (kOpCLds ? opCs : opC)[(rc * TERMS + tt) * 64 + lane];
-#pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
-                const int line = SCF ? j : w, k1 = SCF ? 2 * w + cb : 16 * cb + j;
-                // lane (j, q): z = W512^{n2 k1} Y[k1][n2] for n2 = 4 q + p, p < 4, as
-                // (re, im) pairs: element e of the B fragment is part e & 1 of n2 = 4 q + e / 2
-                const float2 *yp = &ybuf[line * LP + k1 * RP + 4 * q];
-                const float2 *wp = kTwRegs ? twr[cb] : &tws[k1 * TWP + 4 * q];
-                float z[8];
-                float m = 0.f;
-#pragma unroll
-                for (int p = 0; p < 4; p++) {
-                    const float2 y = yp[p], t = wp[p];
-                    z[2 * p] = y.x * t.x - y.y * t.y;
-                    z[2 * p + 1] = y.x * t.y + y.y * t.x;
-                    m = absmax3(m, z[2 * p], z[2 * p + 1]);
-                }
-                m = max_over_groups(m);
-                int ex = 0;
-                (void)frexpf(m, &ex);
-                const float s = ldexpf(1.0f, -ex);
-                vec b[3];
-                if constexpr (PREC == kSplitF16) {
-                    split8_f16_scaled(z, s, b);
-                } else {
-#pragma unroll
-                    for (int e = 0; e < 8; e++) z[e] *= s;
-                    split8<PREC>(z, b);
-                }
-                const f32x4_t xr = split_mma<PREC>(fc[0], b, f32x4_t{});
-                const f32x4_t xi = split_mma<PREC>(fc[1], b, f32x4_t{});
-                const unsigned L = tile * TL + line;
-                if (L < nlines) {
-                    const float sc = ldexpf((float)a.scale, ex);
-                    float2 *__restrict__ p = static_cast<float2 *>(a.out) +
-                                             tile_line_off<AL16, TL>(tp, tile, line, a, true) +
-                                             (long long)(k1 + 32 * q) * a.out_sa;
-                    const long long step = 128 * a.out_sa;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) st_stream(&p[r * step], make_float2(xr[r] * sc, xi[r] * sc));
-                }
-            }
-        }
-    };
-
-    // two register sets: the loads of tile t + gridDim.x go to the set tile t does not use
-    unsigned tile = blockIdx.x;
-    float2 xa[8], xb[8];
-    tile_pos pa = locate_tile<AL16, TL>(tile, a), pb{0, 0};
-    issue(tile, pa, xa);
-    int par = 0;
-    for (;;) {
-        unsigned next = tile + gridDim.x;
-        if (next < ntiles) {
-            pb = locate_tile<AL16, TL>(next, a);
-            issue(next, pb, xb);
-        }
-        run_tile(tile, pa, par, xa);
-        tile = next;
-        par ^= 1;
-        if (tile >= ntiles) break;
-        next = tile + gridDim.x;
-        if (next < ntiles) {
-            pa = locate_tile<AL16, TL>(next, a);
-            issue(next, pa, xa);
-        }
-        run_tile(tile, pb, par, xb);
-        tile = next;
-        par ^= 1;
-        if (tile >= ntiles) break;
     }
 }
 
@@ -493,36 +439,29 @@ std::vector<unsigned char> mfma512_operand_table(int prec, bool conj) {
 }
 
 namespace {
-template <int PREC, bool LCF, bool SCF, int TL>
-void launch_mfma512_al(const pass_args &a, int cus, hipStream_t s) {
-    const long long ntiles = ((long long)a.nlines + TL - 1) / TL;
-    // persistent: one 16-line block or two 8-line blocks per CU, never more blocks than tiles
-    const long long grid = std::min<long long>(ntiles, (long long)std::max(1, cus) * (16 / TL));
-    if ((a.ncol % TL) == 0)
-        fft512_mfma_persistent<PREC, LCF, SCF, true, TL><<<dim3((unsigned)grid), dim3(64 * TL), 0, s>>>(a, (unsigned)ntiles);
+template <int PREC, bool LCF, bool SCF> void launch_mfma512_al(const pass_args &a, long long ntiles, hipStream_t s) {
+    if ((a.ncol % 16) == 0)
+        fft512_mfma<PREC, LCF, SCF, true><<<dim3((unsigned)ntiles), dim3(1024), 0, s>>>(a);
     else
-        fft512_mfma_persistent<PREC, LCF, SCF, false, TL><<<dim3((unsigned)grid), dim3(64 * TL), 0, s>>>(a, (unsigned)ntiles);
+        fft512_mfma<PREC, LCF, SCF, false><<<dim3((unsigned)ntiles), dim3(1024), 0, s>>>(a);
 }
-template <int PREC> void launch_mfma512_prec(const pass_args &a, int cus, hipStream_t s) {
+template <int PREC> void launch_mfma512_prec(const pass_args &a, long long ntiles, hipStream_t s) {
     if (a.load_colfast) {
-        if (a.store_colfast) launch_mfma512_al<PREC, true, true, 16>(a, cus, s);
-        else launch_mfma512_al<PREC, true, false, 16>(a, cus, s);
+        if (a.store_colfast) launch_mfma512_al<PREC, true, true>(a, ntiles, s);
+        else launch_mfma512_al<PREC, true, false>(a, ntiles, s);
     } else {
-        if (a.store_colfast) launch_mfma512_al<PREC, false, true, 16>(a, cus, s);
-        else launch_mfma512_al<PREC, false, false, MFMA_TL_CONTIG>(a, cus, s);
+        if (a.store_colfast) launch_mfma512_al<PREC, false, true>(a, ntiles, s);
+        else launch_mfma512_al<PREC, false, false>(a, ntiles, s);
     }
 }
 }  // namespace
 
 hipError_t launch_mfma512(int prec, const pass_args &a, long long nblocks, hipStream_t s) {
     if (nblocks <= 0) return hipSuccess;
-    if (prec != kSplitF16 && prec != kSplitBF16) return hipErrorInvalidValue;
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return e;
-    if (prec == kSplitF16) launch_mfma512_prec<kSplitF16>(a, cus, s);
-    else launch_mfma512_prec<kSplitBF16>(a, cus, s);
+    if (nblocks > 0xffffffffLL) return hipErrorInvalidValue;
+    if (prec == kSplitF16) launch_mfma512_prec<kSplitF16>(a, nblocks, s);
+    else if (prec == kSplitBF16) launch_mfma512_prec<kSplitBF16>(a, nblocks, s);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -104,7 +104,7 @@ int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream);
  *   HMI_PRECISION_FP32       vector-ALU FP32 (default)
  *   HMI_PRECISION_SPLIT_FP16 512-point passes as 32x16 DFT GEMMs on MFMA, operands split
  *                            into two FP16 terms (3 products), FP32 accumulation,
- *                            per-line power-of-two scaling
+ *                            a power-of-two scale per 32- or 16-point DFT
  *   HMI_PRECISION_SPLIT_BF16 the same with three BF16 terms (6 products)
  * Other lengths and complex<double> always use the vector-ALU passes. */
 #define HMI_PRECISION_FP32 0

@@ -1,7 +1,8 @@
 """GPU parity of the split-precision MFMA form of the 512-point complex<float> passes
 (HMI_PRECISION_SPLIT_FP16: two FP16 terms, 3 products; HMI_PRECISION_SPLIT_BF16: three BF16
-terms, 6 products; FP32 accumulation, per-line power-of-two scaling) against the FP64 oracle,
-bound 5e-6 relative L2, including inputs with 6 decades of dynamic range."""
+terms, 6 products; FP32 accumulation, a power-of-two scale per 32- or 16-point DFT) against the
+FP64 oracle, bound 5e-6 relative L2, including inputs with 6 decades of dynamic range, partial
+tiles (a line count that is not a multiple of 16) and batches."""
 import numpy as np
 import pytest
 
@@ -15,7 +16,9 @@ pytestmark = pytest.mark.gpu
                                                    ((512, 24, 16), (0, 1, 2), (2, 0, 1)),
                                                    ((16, 512, 20), (1, 2, 0), (0, 1, 2)),
                                                    ((8, 20, 512), (2, 1, 0), (1, 0, 2)),
-                                                   ((512, 512, 4), (0, 1, 2), (1, 2, 0))])
+                                                   ((512, 512, 4), (0, 1, 2), (1, 2, 0)),
+                                                   ((512, 7, 3), (0, 1, 2), (1, 0, 2)),
+                                                   ((5, 512, 3), (1, 0, 2), (0, 2, 1))])
 def test_split_forms_vs_oracle(hmi, cuda, oracle, form, size, inorder, outorder):
     import torch
     n = int(np.prod(size))
@@ -42,7 +45,7 @@ def test_split_forms_vs_oracle(hmi, cuda, oracle, form, size, inorder, outorder)
 
 @pytest.mark.parametrize("form", ["split_fp16", "split_bf16"])
 def test_split_forms_dynamic_range(hmi, cuda, oracle, form):
-    """Lines whose magnitudes span 6 decades (1e-3 .. 1e3): the per-line scaling keeps
+    """Elements whose magnitudes span 6 decades (1e-3 .. 1e3): the per-DFT scaling keeps
     both split terms in range."""
     import torch
     size = (512, 32, 8)
@@ -114,3 +117,28 @@ def test_power_law_field(hmi, cuda, oracle, form):
     print(form, f"|k|^-5/3 field forward rel L2 {err:.2e}, max {mx:.2e}")
     assert err <= 5e-6
     assert mx <= 5e-6
+
+
+@pytest.mark.parametrize("form", ["split_fp16", "split_bf16"])
+def test_split_forms_batch(hmi, cuda, oracle, form):
+    """batch = 3 transforms of 512 x 9 x 5 (45 lines per axis pass and transform: partial
+    tiles), out-of-place forward and in-place backward."""
+    import torch
+    size, batch = (512, 9, 5), 3
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (0, 1, 2))
+    outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (2, 0, 1))
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    plan.set_precision(getattr(hmi.precision, form))
+    rng = np.random.default_rng(21)
+    x = rng.standard_normal(batch * n) + 1j * rng.standard_normal(batch * n)
+    xr = x.astype(np.complex64).astype(np.complex128)
+    tx = fc.to_gpu(x, np.complex64)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty, batch=batch)
+    bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size], (0, 1, 2))
+    bo = oracle.Box.make((0, 0, 0), [s - 1 for s in size], (2, 0, 1))
+    ref = oracle.fft3d_c2c(bi, bo, xr, -1, 1.0, batch)
+    assert fc.rel_l2(fc.to_host(ty), ref) <= 5e-6
+    plan.backward(ty, ty, hmi.scale.full, batch=batch)
+    assert fc.rel_l2(fc.to_host(ty), xr) <= 5e-6

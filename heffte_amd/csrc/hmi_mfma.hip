@@ -34,7 +34,6 @@ namespace {
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
-typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 
 template <int PREC> struct split_traits;
 
@@ -54,13 +53,6 @@ template <> struct split_traits<kSplitBF16> {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
-
-// sign flip of eight 16-bit values (FP16 and BF16 keep the sign in bit 15)
-template <typename V> __device__ __forceinline__ V neg8(V v) {
-    u32x4_t u = __builtin_bit_cast(u32x4_t, v);
-    u ^= 0x80008000u;
-    return __builtin_bit_cast(V, u);
-}
 
 // split 8 FP32 values into TERMS low-precision vectors
 template <int PREC>

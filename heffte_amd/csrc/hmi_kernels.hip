@@ -667,7 +667,9 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
     X(double, 32, 8, 32, 3, 4, 8)                \
     X(double, 64, 8, 16, 3, 8, 8)                \
     X(double, 128, 16, 16, 4, 16, 8)             \
-    X(double, 256, 16, 8, 4, 16, 16)             \
+    X(double, 256, 8, 16, 3, 8, 8, 4)            \
+    X(double, 256, 8, 8, 3, 8, 8, 4)             \
+    X(double, 256, 8, 4, 3, 8, 8, 4)             \
     X(double, 512, 8, 8, 3, 8, 8, 8)             \
     X(double, 512, 8, 4, 3, 8, 8, 8)             \
     X(double, 1024, 16, 8, 4, 16, 8, 8)          \

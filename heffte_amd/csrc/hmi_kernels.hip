@@ -525,8 +525,12 @@ __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const ge
 #undef HMI_BATCH_CASE
 }
 
+// The kernel carries every tile of the table, so its register count is the largest tile's
+// (215 VGPRs for complex<float> unconstrained: 2 waves per SIMD); capping it at 3 waves per
+// SIMD (2 for complex<double>) spills no VGPRs and runs the 1000-transform batch 6.5 % faster.
 template <typename R>
-__global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256) fft_batched(batch_args b) {
+__global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256)
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) fft_batched(batch_args b) {
     using C = typename cvec<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *smem = reinterpret_cast<C *>(smem_raw);

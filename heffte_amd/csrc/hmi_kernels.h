@@ -41,9 +41,14 @@ struct batch_kernel_info {
 };
 // Returns the tile id for length n (or -1: the runtime-plan tile runs it).
 int find_batch_kernel(int n, batch_kernel_info *info);
+// Register class of a tile id: 0 for the compile-time tiles with at most 12 elements per
+// thread, 1 for the others and the runtime-plan tile (-1).  Each class has its own build of
+// the batched kernel, so the light tiles are not held to the heavy tiles' register count.
+int batch_kernel_class(int kid);
 
-// One phase of a batch of mixed-size c2c transforms (see batch_args).
-hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
+// One phase (one axis) of a batch of mixed-size c2c transforms (see batch_args), for the
+// work items of one register class.
+hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
 
 // Precision forms of the c2c passes: plain FP32/FP64 vector ALU, or the 512-point
 // complex<float> pass on the matrix cores with split-precision operands.

@@ -504,6 +504,11 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
     (size_t)(T + 1) * (N + (N >> PS) + 1) * 8, (size_t)(T / 2 + 1) * (N + (N >> PS) + 1) * 16},
 static const batch_kernel_info kBatch[] = {HMI_BATCH_TABLE(HMI_BATCH_INFO)};
 
+#define HMI_BATCH_CLS(K, N, E, T, PS, ...) ((E) <= 12 ? 0 : 1),
+static const int kBatchClass[] = {HMI_BATCH_TABLE(HMI_BATCH_CLS)};
+
+int batch_kernel_class(int kid) { return kid < 0 ? 1 : kBatchClass[kid]; }
+
 int find_batch_kernel(int n, batch_kernel_info *info) {
     for (int k = 0; k < (int)(sizeof(kBatch) / sizeof(kBatch[0])); k++)
         if (kBatch[k].n == n) {
@@ -513,25 +518,32 @@ int find_batch_kernel(int n, batch_kernel_info *info) {
     return -1;
 }
 
-template <typename R>
+template <typename R, int CLS>
 __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const generic_plan &g, unsigned tile,
                                            typename cvec<R>::type *smem) {
     constexpr bool D = sizeof(R) == 8;
-#define HMI_BATCH_CASE(K, N, E, T, PS, ...) \
-    case K: uniform_tile<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem); break;
+#define HMI_BATCH_CASE(K, N, E, T, PS, ...)                                                        \
+    case K:                                                                                        \
+        if constexpr (((E) <= 12 ? 0 : 1) == CLS)                                                  \
+            uniform_tile<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem);          \
+        break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_BATCH_CASE)
-        default: generic_tile<R>(a, g, tile, smem);
+        default:
+            if constexpr (CLS == 1) generic_tile<R>(a, g, tile, smem);
     }
 #undef HMI_BATCH_CASE
 }
 
-// The kernel carries every tile of the table, so its register count is the largest tile's
-// (215 VGPRs for complex<float> unconstrained: 2 waves per SIMD); capping it at 3 waves per
-// SIMD (2 for complex<double>) spills no VGPRs and runs the 1000-transform batch 6.5 % faster.
-template <typename R>
+// One build per register class CLS (batch_kernel_class): a build carries every tile of its
+// class, so its register count is its largest tile's.  Occupancy caps without VGPR spills:
+// light class 5 waves per SIMD (complex<float>) / 4 (complex<double>), heavy class (up to 21
+// elements per thread and the runtime-plan tile) 3 / 2.  Splitting the classes took the
+// 1000-transform batch from 11.4 to 10.4 ms.
+template <typename R, int CLS>
 __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256)
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) fft_batched(batch_args b) {
+__attribute__((amdgpu_waves_per_eu(CLS == 0 ? (sizeof(R) == 8 ? 4 : 5) : (sizeof(R) == 8 ? 2 : 3))))
+fft_batched(batch_args b) {
     using C = typename cvec<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *smem = reinterpret_cast<C *>(smem_raw);
@@ -566,22 +578,29 @@ __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) fft_batched(batch_a
         a.mul = nullptr;
         const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
         a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
-        batch_tile<R>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);
+        batch_tile<R, CLS>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);
         __syncthreads();
     }
 }
 
-hipError_t launch_batched(bool dbl, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s) {
+hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s) {
     static bool configured = false;
     if (!configured) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched<float>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched<double>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        const void *fns[] = {reinterpret_cast<const void *>(&fft_batched<float, 0>),
+                             reinterpret_cast<const void *>(&fft_batched<float, 1>),
+                             reinterpret_cast<const void *>(&fft_batched<double, 0>),
+                             reinterpret_cast<const void *>(&fft_batched<double, 1>)};
+        for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
         configured = true;
     }
-    if (dbl) fft_batched<double><<<dim3((unsigned)nblocks), dim3(128), lds_bytes, s>>>(b);
-    else fft_batched<float><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(b);
+    const dim3 grid((unsigned)nblocks);
+    if (dbl) {
+        if (cls == 0) fft_batched<double, 0><<<grid, dim3(128), lds_bytes, s>>>(b);
+        else fft_batched<double, 1><<<grid, dim3(128), lds_bytes, s>>>(b);
+    } else {
+        if (cls == 0) fft_batched<float, 0><<<grid, dim3(256), lds_bytes, s>>>(b);
+        else fft_batched<float, 1><<<grid, dim3(256), lds_bytes, s>>>(b);
+    }
     return hipGetLastError();
 }
 

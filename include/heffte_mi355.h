@@ -176,8 +176,9 @@ int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const
  * heFFTe batches share one geometry, include/heffte_fft3d.h:392-402).  Each transform is a
  * c2c box of sizes[3*i .. 3*i+2] in order (0, 1, 2) stored at element offset in_offsets[i]
  * of the input array and out_offsets[i] of the output array (NULL = packed back to back;
- * out_offsets NULL = same as in_offsets).  The forward/backward calls run three launches
- * (one per axis) over a device work list, replayed from a captured hipGraph unless
+ * out_offsets NULL = same as in_offsets).  The forward/backward calls run three phases
+ * (one per axis; each one launch per register class of tile) over device work lists, replayed
+ * from a captured hipGraph unless
  * heffte_mi355_batch_set_graph(batch, 0) was called.  scale applies per transform (1/N_i). */
 typedef struct heffte_mi355_batch_s *heffte_mi355_batch;
 

@@ -213,7 +213,16 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             }
         } else {
             const C *__restrict__ src = static_cast<const C *>(a.in);
-            if (a.mul) {
+            if (a.mul == a.in) {
+                // squared input (convolution of an array with itself): one load per element
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const long long off = base + (long long)(t + P * e) * a.in_sa;
+                    C y = valid ? ldg<NT>(&src[off]) : cmk<C>(R(0), R(0));
+                    C x = cmul(y, y);
+                    v[e] = cmk<C>(x.x, sg * x.y);
+                }
+            } else if (a.mul) {
                 // pointwise product with a second array of the same layout (fused convolution)
                 const C *__restrict__ mul = static_cast<const C *>(a.mul);
 #pragma unroll
@@ -391,7 +400,7 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
                 x.y *= -sg;
             } else {
                 x = static_cast<const C *>(a.in)[off];
-                if (a.mul) x = cmul(x, static_cast<const C *>(a.mul)[off]);
+                if (a.mul) x = cmul(x, a.mul == a.in ? x : static_cast<const C *>(a.mul)[off]);
                 x.y *= sg;
             }
         }

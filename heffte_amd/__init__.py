@@ -65,7 +65,7 @@ def _load():
     lib.heffte_mi355_reshape.argtypes = [ctypes.c_int, LL3, LL3, I3, I3, ctypes.c_int, vp, vp, vp]
     lib.heffte_mi355_size_convolve_workspace.argtypes = [plan_p]
     lib.heffte_mi355_size_convolve_workspace.restype = ctypes.c_longlong
-    for f in ("convolve_c2c", "convolve_z2z"):
+    for f in ("convolve_c2c", "convolve_z2z", "convolve_spectrum_c2c", "convolve_spectrum_z2z"):
         fn = getattr(lib, "heffte_mi355_" + f)
         fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, vp, ctypes.c_int]
         fn.restype = ctypes.c_int
@@ -102,6 +102,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
     "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
     "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
+    "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z",
 )
 
 
@@ -286,6 +287,28 @@ class fft_plan:
             raise heffte_input_error("convolve() called with insufficient workspace size")
         fn = getattr(lib, f"heffte_mi355_convolve_{key}")
         _check(fn(self._h, batch, _ptr(x), _ptr(y), _ptr(outarray), _ptr(workspace), scaling), "convolve")
+
+    def convolve_spectrum(self, x, spectrum, outarray, workspace=None, scaling=scale.full, batch=1):
+        """outarray = backward(forward(x) * spectrum) for a spectrum = forward(y) (scale.none,
+        outbox layout) computed once by the caller: one forward and one backward transform per
+        call.  outarray may alias x, not spectrum; workspace as for forward_buffered."""
+        if self.kind != kind.c2c:
+            raise heffte_input_error("convolve_spectrum() needs a c2c plan")
+        if scaling not in (0, 1, 2):
+            raise heffte_input_error("convolve_spectrum() called with invalid scaling")
+        names = {_dtype_name(x), _dtype_name(spectrum), _dtype_name(outarray)}
+        if len(names) != 1 or names.copy().pop() not in ("complex64", "complex128"):
+            raise heffte_input_error(f"convolve_spectrum() called with incompatible types {sorted(names)}")
+        key = "c2c" if _dtype_name(x) == "complex64" else "z2z"
+        if x.numel() < batch * self.size_inbox() or spectrum.numel() < batch * self.size_outbox() or \
+                outarray.numel() < batch * max(self.size_inbox(), self.size_outbox()):
+            raise heffte_input_error("convolve_spectrum() called with invalid array size")
+        if workspace is not None and workspace.numel() * workspace.element_size() < \
+                batch * self.size_workspace() * (8 if key == "c2c" else 16):
+            raise heffte_input_error("convolve_spectrum() called with insufficient workspace size")
+        fn = getattr(lib, f"heffte_mi355_convolve_spectrum_{key}")
+        _check(fn(self._h, batch, _ptr(x), _ptr(spectrum), _ptr(outarray), _ptr(workspace), scaling),
+               "convolve_spectrum")
 
 
 def _check_backend(backend_tag):

@@ -92,6 +92,14 @@ int run_checked(heffte_mi355_plan plan, bool r2r, int dir, bool dbl, bool in_rea
     return run(plan, dir, dbl, in_real, out_real, batch, in, out, ws, scale);
 }
 
+int convolve_spectrum(heffte_mi355_plan plan, bool dbl, int batch, const void *x, const void *yhat, void *out,
+                      void *ws, int scale) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    if (plan->impl->get_kind() != hmi::kind::c2c) return fail(HMI_ERROR_INVALID_ARGUMENT, "convolution needs a c2c plan");
+    if (scale < 0 || scale > 2) return fail(HMI_ERROR_INVALID_ARGUMENT, "scale must be 0, 1 or 2");
+    return guarded([&] { plan->impl->convolve_spectrum(dbl, batch, x, yhat, out, ws, scale); });
+}
+
 int convolve(heffte_mi355_plan plan, bool dbl, int batch, const void *x, const void *y, void *out, void *ws,
              int scale) {
     if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
@@ -206,6 +214,14 @@ int heffte_mi355_convolve_c2c(heffte_mi355_plan p, int b, void const *x, void co
 }
 int heffte_mi355_convolve_z2z(heffte_mi355_plan p, int b, void const *x, void const *y, void *out, void *ws, int s) {
     return convolve(p, true, b, x, y, out, ws, s);
+}
+int heffte_mi355_convolve_spectrum_c2c(heffte_mi355_plan p, int b, void const *x, void const *yhat, void *out,
+                                       void *ws, int s) {
+    return convolve_spectrum(p, false, b, x, yhat, out, ws, s);
+}
+int heffte_mi355_convolve_spectrum_z2z(heffte_mi355_plan p, int b, void const *x, void const *yhat, void *out,
+                                       void *ws, int s) {
+    return convolve_spectrum(p, true, b, x, yhat, out, ws, s);
 }
 int heffte_mi355_forward_s2s(heffte_mi355_plan p, int b, float const *in, float *out, void *ws, int s) {
     return run_checked(p, true, -1, false, true, true, b, in, out, ws, s);

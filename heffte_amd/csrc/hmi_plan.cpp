@@ -614,6 +614,16 @@ void fft_plan::convolve(bool dbl, int batch, const void *x, const void *y, void 
     if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
 }
 
+void fft_plan::convolve_spectrum(bool dbl, int batch, const void *x, const void *yhat, void *out, void *ws,
+                                 int scaling) {
+    if (kind_ != kind::c2c) throw std::invalid_argument("convolution needs a c2c plan");
+    if (batch < 1) throw std::invalid_argument("batch size must be positive");
+    if (!x || !yhat || !out) throw std::invalid_argument("null input or output array");
+    if (out == yhat) throw std::invalid_argument("the output must not alias the spectrum");
+    execute(-1, dbl, false, false, batch, x, out, ws, 0);
+    execute(+1, dbl, false, false, batch, out, out, ws, scaling, yhat);
+}
+
 void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,
              hipStream_t s) {
     if (!is_permutation(inbox.order) || !is_permutation(out_order))

@@ -104,6 +104,10 @@ class fft_plan {
     // c2c plans only; ws holds batch * convolve_workspace() elements or is null (allocated).
     void convolve(bool dbl, int batch, const void *x, const void *y, void *out, void *ws, int scaling);
     long long convolve_workspace() const { return ws_elems_ + outbox_.count(); }
+    // out = backward(forward(x) * yhat) with yhat a spectrum the caller computed once (forward
+    // of the second operand, outbox layout, no scale): one forward and one backward per call.
+    // ws holds batch * size_workspace() elements or is null; out must not alias yhat.
+    void convolve_spectrum(bool dbl, int batch, const void *x, const void *yhat, void *out, void *ws, int scaling);
 
     // Plan (without running it): exposed for tests of the planner.
     const exec_plan &get_exec_plan(int dir, bool dbl, bool in_real, bool out_real, bool in_place);

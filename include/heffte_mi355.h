@@ -151,6 +151,15 @@ int heffte_mi355_convolve_c2c(heffte_mi355_plan plan, int batch, void const *x, 
                               void *workspace, int scale);
 int heffte_mi355_convolve_z2z(heffte_mi355_plan plan, int batch, void const *x, void const *y, void *output,
                               void *workspace, int scale);
+/* The same with the second operand's spectrum computed once by the caller (a filter applied to
+ * many inputs): spectrum = forward(y) with HMI_SCALE_NONE, in the outbox layout, and
+ *   output = backward(forward(x) * spectrum)
+ * runs one forward and one backward transform.  workspace holds batch *
+ * heffte_mi355_size_workspace(plan) elements or is NULL; output may alias x, not spectrum. */
+int heffte_mi355_convolve_spectrum_c2c(heffte_mi355_plan plan, int batch, void const *x, void const *spectrum,
+                                       void *output, void *workspace, int scale);
+int heffte_mi355_convolve_spectrum_z2z(heffte_mi355_plan plan, int batch, void const *x, void const *spectrum,
+                                       void *output, void *workspace, int scale);
 
 /* Real-to-real transforms of an HMI_KIND_COS/SIN/COS1 plan; replace
  * rtransform::forward/backward(float/double data, workspace)

@@ -163,3 +163,27 @@ def test_convolve_rejections(hmi, cuda):
     with pytest.raises(hmi.heffte_input_error):
         plan.convolve(x, x, x, workspace=torch.zeros(4, dtype=torch.complex64, device=cuda))
     assert r2c.size_convolve_workspace() == -1
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_convolve_spectrum(hmi, cuda, oracle, dtype):
+    """A precomputed spectrum (forward of y, scale none) gives the same result as convolve(x, y),
+    also in place on x and with batch = 2."""
+    import torch
+    size, batch = (12, 10, 9), 2
+    n = int(np.prod(size))
+    plan = make_plan(hmi, size, (0, 1, 2), (2, 1, 0))
+    x = np.concatenate([fc.uniform_complex(n, 40 + b) for b in range(batch)])
+    y = np.concatenate([fc.uniform_complex(n, 50 + b) for b in range(batch)])
+    tx, ty = fc.to_gpu(x, CDT[dtype]), fc.to_gpu(y, CDT[dtype])
+    yhat = torch.empty_like(ty)
+    plan.forward(ty, yhat, batch=batch)
+    out = torch.empty_like(tx)
+    plan.convolve_spectrum(tx, yhat, out, batch=batch)
+    ref = oracle_conv(oracle, size, (0, 1, 2), (2, 1, 0), x, y, 1.0 / n, batch)
+    assert fc.rel_l2(fc.to_host(out), ref) <= fc.TOL[dtype]
+    ws = torch.empty(batch * max(1, plan.size_workspace()), dtype=tx.dtype, device=cuda)
+    plan.convolve_spectrum(tx, yhat, tx, ws, hmi.scale.full, batch=batch)
+    assert fc.rel_l2(fc.to_host(tx), ref) <= fc.TOL[dtype]
+    with pytest.raises(hmi.heffte_input_error):
+        plan.convolve_spectrum(ty, yhat, yhat, batch=batch)

@@ -90,7 +90,7 @@ void batch_plan::prepare(bool dbl) {
     std::vector<int> twoff, kids;
     std::vector<float> tf;
     std::vector<double> td;
-    size_t lds[2] = {0, 0};
+    size_t lds[kBatchClasses] = {};
     for (int n : lengths_) {
         auto fc = forced_.find(n);
         const batch_choice c = fc != forced_.end() ? fc->second
@@ -107,7 +107,8 @@ void batch_plan::prepare(bool dbl) {
         } else {
             const size_t per_line = 2 * (size_t)n * elem;
             g = make_generic_plan(n, std::max(1, c.lines), c.radices.empty() ? default_radices(n) : c.radices);
-            lds[1] = std::max(lds[1], per_line * g.lines);
+            size_t &l = lds[batch_kernel_class(-1)];
+            l = std::max(l, per_line * g.lines);
         }
         if (dbl) {
             twoff.push_back((int)(td.size() / 2));
@@ -123,7 +124,7 @@ void batch_plan::prepare(bool dbl) {
     }
     // work lists: every tile of every transform, one list per axis and register class
     for (int ax = 0; ax < 3; ax++) {
-        std::vector<batch_item> items[2];
+        std::vector<batch_item> items[kBatchClasses];
         for (int i = 0; i < (int)xf_.size(); i++) {
             const batch_xform &x = xf_[i];
             const int col = (ax == 0) ? 1 : 0, b1 = 3 - ax - col;
@@ -133,7 +134,7 @@ void batch_plan::prepare(bool dbl) {
             auto &list = items[batch_kernel_class(kids[x.size_id[ax]])];
             for (long long t = 0; t < ntiles; t++) list.push_back({i, (int)t});
         }
-        for (int c = 0; c < 2; c++) {
+        for (int c = 0; c < kBatchClasses; c++) {
             if (items[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
             d.nitems[ax][c] = (int)items[c].size();
             d.items[ax][c] = items[c].empty() ? nullptr : upload(items[c]);
@@ -145,8 +146,7 @@ void batch_plan::prepare(bool dbl) {
     d.xf = upload(xf_);
     if (dbl) d.tw = upload(td);
     else d.tw = upload(tf);
-    d.lds[0] = lds[0];
-    d.lds[1] = lds[1];
+    for (int c = 0; c < kBatchClasses; c++) d.lds[c] = lds[c];
     d.ready = true;
 }
 
@@ -165,7 +165,7 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
         b.from_in = (ax == 0) ? 1 : 0;
         b.conj = dir > 0 ? 1 : 0;
         b.scaling = (ax == 2) ? scaling : 0;
-        for (int c = 0; c < 2; c++) {
+        for (int c = 0; c < kBatchClasses; c++) {
             b.items = d.items[ax][c];
             b.nitems = d.nitems[ax][c];
             // one work item per block: the hardware schedules the blocks as others retire (a

@@ -57,14 +57,14 @@ class batch_plan {
     struct device_state {
         void *tw = nullptr;
         batch_xform *xf = nullptr;
-        // work lists per axis and register class (0: light compile-time tiles, 1: the others
-        // and the runtime-plan tile), each run by its own build of the batched kernel
-        batch_item *items[3][2] = {};
-        int nitems[3][2] = {};
+        // work lists per axis and register class of tile (batch_kernel_class), each run by its
+        // own build of the batched kernel
+        batch_item *items[3][kBatchClasses] = {};
+        int nitems[3][kBatchClasses] = {};
         generic_plan *plans = nullptr;
         int *twoff = nullptr;
         int *kid = nullptr;
-        size_t lds[2] = {0, 0};
+        size_t lds[kBatchClasses] = {};
         bool ready = false;
     };
     struct graph_key {

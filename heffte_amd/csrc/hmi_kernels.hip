@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 #include "hmi_device.h"
 #include "hmi_kernels.h"
@@ -513,10 +514,18 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
     (size_t)(T + 1) * (N + (N >> PS) + 1) * 8, (size_t)(T / 2 + 1) * (N + (N >> PS) + 1) * 16},
 static const batch_kernel_info kBatch[] = {HMI_BATCH_TABLE(HMI_BATCH_INFO)};
 
-#define HMI_BATCH_CLS(K, N, E, T, PS, ...) ((E) <= 12 ? 0 : 1),
+// Register class of a compile-time tile with E elements per thread (the runtime-plan tile is in
+// the last class), and the waves per SIMD each class's build is held to ({complex<float>,
+// complex<double>}): the highest occupancy at which the largest tile of the class does not spill
+// VGPRs.  A build carries every tile of its class, so its register count is its largest tile's.
+constexpr int batch_class_of(int e) { return e <= 12 ? 0 : (e <= 16 ? 1 : 2); }
+constexpr int kBatchWaves[kBatchClasses][2] = {{5, 4}, {4, 3}, {3, 2}};
+static_assert(batch_class_of(21) == kBatchClasses - 1, "the 21-element tile shares the runtime-plan tile's class");
+
+#define HMI_BATCH_CLS(K, N, E, T, PS, ...) batch_class_of(E),
 static const int kBatchClass[] = {HMI_BATCH_TABLE(HMI_BATCH_CLS)};
 
-int batch_kernel_class(int kid) { return kid < 0 ? 1 : kBatchClass[kid]; }
+int batch_kernel_class(int kid) { return kid < 0 ? kBatchClasses - 1 : kBatchClass[kid]; }
 
 int find_batch_kernel(int n, batch_kernel_info *info) {
     for (int k = 0; k < (int)(sizeof(kBatch) / sizeof(kBatch[0])); k++)
@@ -533,25 +542,23 @@ __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const ge
     constexpr bool D = sizeof(R) == 8;
 #define HMI_BATCH_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                        \
-        if constexpr (((E) <= 12 ? 0 : 1) == CLS)                                                  \
+        if constexpr (batch_class_of(E) == CLS)                                                    \
             uniform_tile<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem);          \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_BATCH_CASE)
         default:
-            if constexpr (CLS == 1) generic_tile<R>(a, g, tile, smem);
+            if constexpr (CLS == kBatchClasses - 1) generic_tile<R>(a, g, tile, smem);
     }
 #undef HMI_BATCH_CASE
 }
 
-// One build per register class CLS (batch_kernel_class): a build carries every tile of its
-// class, so its register count is its largest tile's.  Occupancy caps without VGPR spills:
-// light class 5 waves per SIMD (complex<float>) / 4 (complex<double>), heavy class (up to 21
-// elements per thread and the runtime-plan tile) 3 / 2.  Splitting the classes took the
-// 1000-transform batch from 11.4 to 10.4 ms.
+// One build per register class CLS (batch_kernel_class), held to kBatchWaves[CLS] waves per SIMD.
+// Splitting the tiles into three classes (from one build at 3 / 2 waves per SIMD) took the
+// 1000-transform batch from 11.4 to 6.7 ms per forward + backward pair.
 template <typename R, int CLS>
 __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256)
-__attribute__((amdgpu_waves_per_eu(CLS == 0 ? (sizeof(R) == 8 ? 4 : 5) : (sizeof(R) == 8 ? 2 : 3))))
+__attribute__((amdgpu_waves_per_eu(kBatchWaves[CLS][sizeof(R) == 8])))
 fft_batched(batch_args b) {
     using C = typename cvec<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -592,24 +599,27 @@ fft_batched(batch_args b) {
     }
 }
 
+using batched_fn = void (*)(batch_args);
+
+template <int... C>
+static const batched_fn *batched_builds(bool dbl, std::integer_sequence<int, C...>) {
+    static const batched_fn f[2][sizeof...(C)] = {{&fft_batched<float, C>...}, {&fft_batched<double, C>...}};
+    return f[dbl ? 1 : 0];
+}
+
 hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s) {
-    static bool configured = false;
-    if (!configured) {
-        const void *fns[] = {reinterpret_cast<const void *>(&fft_batched<float, 0>),
-                             reinterpret_cast<const void *>(&fft_batched<float, 1>),
-                             reinterpret_cast<const void *>(&fft_batched<double, 0>),
-                             reinterpret_cast<const void *>(&fft_batched<double, 1>)};
-        for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
-        configured = true;
-    }
-    const dim3 grid((unsigned)nblocks);
-    if (dbl) {
-        if (cls == 0) fft_batched<double, 0><<<grid, dim3(128), lds_bytes, s>>>(b);
-        else fft_batched<double, 1><<<grid, dim3(128), lds_bytes, s>>>(b);
-    } else {
-        if (cls == 0) fft_batched<float, 0><<<grid, dim3(256), lds_bytes, s>>>(b);
-        else fft_batched<float, 1><<<grid, dim3(256), lds_bytes, s>>>(b);
-    }
+    static const bool configured = [] {
+        for (int d = 0; d < 2; d++)
+            for (int c = 0; c < kBatchClasses; c++)
+                (void)hipFuncSetAttribute(
+                    reinterpret_cast<const void *>(batched_builds(d != 0, std::make_integer_sequence<int, kBatchClasses>())[c]),
+                    hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        return true;
+    }();
+    (void)configured;
+    if (cls < 0 || cls >= kBatchClasses) return hipErrorInvalidValue;
+    const batched_fn fn = batched_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
+    fn<<<dim3((unsigned)nblocks), dim3(dbl ? 128 : 256), lds_bytes, s>>>(b);
     return hipGetLastError();
 }
 

@@ -232,11 +232,21 @@ std::string cache_path() {
     return "";
 }
 
+// Choices are timed with the kernels of one build of the library: the cache file starts with
+// this line, which names the build, and a file written by another build is ignored (and
+// replaced by the next save).
+const std::string &cache_header() {
+    static const std::string h =
+        std::string("# heffte_mi355 batch tuning, build ") + __DATE__ " " __TIME__ + " / " + kernels_build_stamp();
+    return h;
+}
+
 std::map<long long, batch_choice> load_cache(const std::string &path) {
     std::map<long long, batch_choice> m;
     if (path.empty()) return m;
     std::ifstream f(path);
     std::string line;
+    if (!std::getline(f, line) || line != cache_header()) return m;
     while (std::getline(f, line)) {
         std::istringstream in(line);
         int n = 0, dbl = 0, table = 0, lines = 0;
@@ -272,6 +282,7 @@ void save_cache(const std::string &path, const std::map<long long, batch_choice>
     {
         std::ofstream f(tmp);
         if (!f) return;  // a read-only location only loses the cache
+        f << cache_header() << '\n';
         for (const auto &kv : m) {
             const batch_choice &c = kv.second;
             f << (kv.first / 2) << ' ' << (kv.first & 1) << ' ' << (c.table ? 1 : 0) << ' ' << c.lines << ' ';

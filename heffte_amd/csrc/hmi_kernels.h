@@ -46,6 +46,9 @@ int find_batch_kernel(int n, batch_kernel_info *info);
 // own build of the batched kernel, so light tiles are not held to heavy tiles' register count.
 constexpr int kBatchClasses = 3;
 int batch_kernel_class(int kid);
+// Compile date and time of the kernels (stamps the autotuner's cache: choices timed with other
+// kernels are not reused).
+const char *kernels_build_stamp();
 
 // One phase (one axis) of a batch of mixed-size c2c transforms (see batch_args), for the
 // work items of one register class.

@@ -527,6 +527,8 @@ static const int kBatchClass[] = {HMI_BATCH_TABLE(HMI_BATCH_CLS)};
 
 int batch_kernel_class(int kid) { return kid < 0 ? kBatchClasses - 1 : kBatchClass[kid]; }
 
+const char *kernels_build_stamp() { return __DATE__ " " __TIME__; }
+
 int find_batch_kernel(int n, batch_kernel_info *info) {
     for (int k = 0; k < (int)(sizeof(kBatch) / sizeof(kBatch[0])); k++)
         if (kBatch[k].n == n) {

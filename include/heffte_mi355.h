@@ -200,7 +200,8 @@ int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
  * distinct side length of the batch gets its radix plan, lines per tile and LDS staging (the
  * compile-time tile or a runtime plan) by timing the candidates on the GPU; choices are cached
  * per (length, precision) in memory and in $HMI_TUNE_CACHE (default
- * $HOME/.cache/heffte_mi355/batch_tuning.txt).  enable = 0 takes the untimed defaults.  The
+ * $HOME/.cache/heffte_mi355/batch_tuning.txt; a file written by another build of the library is
+ * ignored and rewritten).  enable = 0 takes the untimed defaults.  The
  * first call then runs timing launches on a stream of its own: do it outside stream capture. */
 int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable);
 /* Elements (complex) the input / output arrays must hold. */

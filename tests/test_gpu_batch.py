@@ -141,7 +141,9 @@ def test_autotune_cache_and_parity(hmi, cuda, oracle, tmp_path, monkeypatch):
     y = fc.to_host(ty)
     for s, off, r in zip(sizes, offsets, oracle_each(oracle, sizes, offsets, x, -1, 0)):
         assert fc.rel_l2(y[off:off + r.size], r) <= 5e-6, s
-    rows = [line.split() for line in cache.read_text().splitlines() if line.strip()]
+    lines = cache.read_text().splitlines()
+    assert lines[0].startswith("# heffte_mi355 batch tuning, build ")
+    rows = [line.split() for line in lines[1:] if line.strip()]
     tuned = {(int(r[0]), int(r[1])) for r in rows}
     assert {(20, 0), (48, 0), (105, 0), (32, 0), (16, 0)} <= tuned
     for r in rows:
@@ -155,3 +157,26 @@ def test_autotune_cache_and_parity(hmi, cuda, oracle, tmp_path, monkeypatch):
     ty3 = torch.empty_like(tx)
     plan3.forward(tx, ty3)
     assert fc.rel_l2(fc.to_host(ty3), y.astype(np.complex128)) <= 5e-6
+
+
+def test_autotune_cache_from_another_build_is_ignored(hmi, cuda, oracle, tmp_path, monkeypatch):
+    """A cache file whose first line names another build of the library is not read: its choice
+    (a runtime plan of 3 lines for length 20, never a candidate) is replaced by a timed one and
+    the file rewritten under this build's header."""
+    import torch
+    cache = tmp_path / "tuning.txt"
+    cache.write_text("# heffte_mi355 batch tuning, build Jan  1 2000 00:00:00 / Jan  1 2000 00:00:00\n"
+                     "20 0 0 3 2,2,5\n")
+    monkeypatch.setenv("HMI_TUNE_CACHE", str(cache))
+    sizes = [(20, 20, 20)]
+    plan = hmi.batch_plan(sizes)
+    x = fc.uniform_complex(plan.size_input(), 22)
+    tx = torch.from_numpy(x).to(cuda).to(torch.complex64)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    ref = oracle_each(oracle, sizes, [0], x, -1, 0)[0]
+    assert fc.rel_l2(fc.to_host(ty), ref) <= 5e-6
+    lines = cache.read_text().splitlines()
+    assert lines[0].startswith("# heffte_mi355 batch tuning, build ") and "2000" not in lines[0]
+    assert "20 0 0 3 2,2,5" not in lines[1:]
+    assert any(line.split()[:2] == ["20", "0"] for line in lines[1:])

@@ -42,7 +42,7 @@ struct batch_kernel_info {
 // Returns the tile id for length n (or -1: the runtime-plan tile runs it).
 int find_batch_kernel(int n, batch_kernel_info *info);
 // Register class of a tile id: 0 for the compile-time tiles with at most 12 elements per
-// thread, 1 for 14 to 16, 2 for the others and the runtime-plan tile (-1).  Each class has its
+// thread and the staged tiles, 1 for 14 to 16, 2 for the others and the runtime-plan tile (-1).  Each class has its
 // own build of the batched kernel, so light tiles are not held to heavy tiles' register count.
 constexpr int kBatchClasses = 3;
 int batch_kernel_class(int kid);

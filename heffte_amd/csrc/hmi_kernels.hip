@@ -477,6 +477,65 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
     }
 }
 
+// ------------------------------------------------------ staged compile-time tile
+
+template <typename RL, int T, int NTH, int S, typename C>
+__device__ __forceinline__ void staged_stages(const C *src, C *dst, const C *__restrict__ tw) {
+    constexpr int Rr = RL::at(S), N = RL::n(), NB = N / Rr, Ls = RL::ls(S), TWO = RL::twoff(S);
+    for (int idx = threadIdx.x; idx < T * NB; idx += NTH) {
+        const int l = idx / NB, b = idx - l * NB;
+        const int k = b % Ls;
+        generic_bfly<Rr>(src + l * N + b, NB, dst + l * N + (b / Ls) * Ls * Rr + k, Ls, tw + TWO + k * (Rr - 1), k != 0);
+    }
+    __syncthreads();
+    if constexpr (S + 1 < RL::count) staged_stages<RL, T, NTH, S + 1>(dst, const_cast<C *>(src), tw);
+}
+
+// One tile (T lines) of an n-point c2c pass of the batched kernel for a length whose radices
+// have no small common multiple (105 = 3 * 5 * 7: a register tile would hold 105 elements per
+// thread).  Every stage runs one butterfly per work unit (T * n / radix of them) over the
+// NTH threads, from one LDS buffer to the other; radices, lines and strides are compile-time
+// constants and each line's offsets are computed once.  Same Stockham stages and twiddle
+// table as the register tiles.  LDS: 2 * T * n values + 2 * T offsets.
+template <typename R, typename RL, int T, int NTH>
+__device__ __forceinline__ void staged_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *smem) {
+    using C = typename cvec<R>::type;
+    constexpr int N = RL::n();
+    C *buf0 = smem;
+    C *buf1 = buf0 + T * N;
+    long long *base = reinterpret_cast<long long *>(buf1 + T * N);  // [0, T) loads, [T, 2T) stores
+    const int tid = threadIdx.x;
+    if (tid < T) {
+        const unsigned L = tile * T + tid;
+        const bool valid = L < (unsigned)a.nlines;
+        base[tid] = valid ? line_offset(L, a, false) : -1;
+        base[T + tid] = valid ? line_offset(L, a, true) : -1;
+    }
+    __syncthreads();
+    const R sg = a.conj ? R(-1) : R(1);
+    const C *__restrict__ src = static_cast<const C *>(a.in);
+    for (int idx = tid; idx < T * N; idx += NTH) {
+        int l, j;
+        if (a.load_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
+        const long long b = base[l];
+        C x = b >= 0 ? src[b + (long long)j * a.in_sa] : cmk<C>(R(0), R(0));
+        buf0[l * N + j] = cmk<C>(x.x, sg * x.y);
+    }
+    __syncthreads();
+    staged_stages<RL, T, NTH, 0>(buf0, buf1, static_cast<const C *>(a.tw));
+    const C *res = (RL::count % 2) ? buf1 : buf0;
+    const R sc = (R)a.scale;
+    C *__restrict__ dst = static_cast<C *>(a.out);
+    for (int idx = tid; idx < T * N; idx += NTH) {
+        int l, j;
+        if (a.store_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
+        const long long b = base[T + l];
+        if (b < 0) continue;
+        const C y = res[l * N + j];
+        dst[b + (long long)j * a.out_sa] = cmk<C>(y.x * sc, sg * y.y * sc);
+    }
+}
+
 template <typename R>
 __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -492,6 +551,7 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
 // Compile-time tiles of the batched kernel: (n, E, lines T for complex<float>, pad shift,
 // radices).  P = n / E threads per line; P * T <= 256 (complex<float>, 256-thread blocks) and
 // P * (T / 2) <= 128 (complex<double>, 128-thread blocks); LDS = (T + 1) rows stays <= 40 KiB.
+// E = 0 marks a staged tile (staged_tile: one butterfly per work unit and stage, 2 T rows of LDS).
 #define HMI_BATCH_TABLE(X)            \
     X(0, 16, 16, 256, 4, 16)          \
     X(1, 20, 10, 128, 4, 2, 2, 5)     \
@@ -508,16 +568,21 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
     X(12, 80, 10, 32, 4, 2, 2, 2, 2, 5) \
     X(13, 96, 12, 32, 4, 4, 4, 2, 3)  \
     X(14, 112, 14, 32, 4, 2, 2, 2, 2, 7) \
-    X(15, 128, 16, 32, 4, 16, 8)
+    X(15, 128, 16, 32, 4, 16, 8)       \
+    X(16, 105, 0, 16, 0, 3, 5, 7)
 
+constexpr size_t batch_lds_bytes(int n, int e, int t, int ps, size_t value) {
+    return e == 0 ? (size_t)2 * t * n * value + (size_t)2 * t * 8 : (size_t)(t + 1) * (n + (n >> ps) + 1) * value;
+}
 #define HMI_BATCH_INFO(K, N, E, T, PS, ...) {N, T, T / 2, {__VA_ARGS__}, radices<__VA_ARGS__>::count, \
-    (size_t)(T + 1) * (N + (N >> PS) + 1) * 8, (size_t)(T / 2 + 1) * (N + (N >> PS) + 1) * 16},
+    batch_lds_bytes(N, E, T, PS, 8), batch_lds_bytes(N, E, T / 2, PS, 16)},
 static const batch_kernel_info kBatch[] = {HMI_BATCH_TABLE(HMI_BATCH_INFO)};
 
 // Register class of a compile-time tile with E elements per thread (the runtime-plan tile is in
 // the last class), and the waves per SIMD each class's build is held to ({complex<float>,
 // complex<double>}): the highest occupancy at which the largest tile of the class does not spill
 // VGPRs.  A build carries every tile of its class, so its register count is its largest tile's.
+// The staged tile (E = 0) keeps one butterfly in registers at a time: light class.
 constexpr int batch_class_of(int e) { return e <= 12 ? 0 : (e <= 16 ? 1 : 2); }
 constexpr int kBatchWaves[kBatchClasses][2] = {{5, 4}, {4, 3}, {3, 2}};
 static_assert(batch_class_of(21) == kBatchClasses - 1, "the 21-element tile shares the runtime-plan tile's class");
@@ -538,6 +603,12 @@ int find_batch_kernel(int n, batch_kernel_info *info) {
     return -1;
 }
 
+template <typename R, typename RL, int E, int T, int PS>
+__device__ __forceinline__ void batch_tile_of(const pass_args &a, unsigned tile, typename cvec<R>::type *smem) {
+    if constexpr (E == 0) staged_tile<R, RL, T, sizeof(R) == 8 ? 128 : 256>(a, tile, smem);
+    else uniform_tile<R, RL, E, T, PS>(a, tile, smem);
+}
+
 template <typename R, int CLS>
 __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const generic_plan &g, unsigned tile,
                                            typename cvec<R>::type *smem) {
@@ -545,7 +616,7 @@ __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const ge
 #define HMI_BATCH_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                        \
         if constexpr (batch_class_of(E) == CLS)                                                    \
-            uniform_tile<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem);          \
+            batch_tile_of<R, radices<__VA_ARGS__>, E, (D ? T / 2 : T), PS>(a, tile, smem);         \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_BATCH_CASE)
@@ -557,7 +628,8 @@ __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const ge
 
 // One build per register class CLS (batch_kernel_class), held to kBatchWaves[CLS] waves per SIMD.
 // Splitting the tiles into three classes (from one build at 3 / 2 waves per SIMD) took the
-// 1000-transform batch from 11.4 to 6.7 ms per forward + backward pair.
+// 1000-transform batch from 11.4 to 6.7 ms per forward + backward pair; the staged tile of
+// length 105 (in place of the runtime-plan tile) to 5.9 ms.
 template <typename R, int CLS>
 __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256)
 __attribute__((amdgpu_waves_per_eu(kBatchWaves[CLS][sizeof(R) == 8])))

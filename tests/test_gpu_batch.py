@@ -53,6 +53,29 @@ def test_batch_vs_oracle(hmi, cuda, oracle, dtype, graph):
 
 
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+@pytest.mark.parametrize("autotune", [False, True])
+def test_batch_length_105_tiles(hmi, cuda, oracle, dtype, autotune):
+    """Length 105 = 3 * 5 * 7 on every axis, with line counts that are not multiples of the tile's
+    lines (16 / 8): the staged compile-time tile (untimed default) and the tuner's choice."""
+    import torch
+    sizes = [(105, 7, 3), (5, 105, 9), (3, 2, 105), (105, 105, 2), (105, 1, 1), (1, 105, 105)]
+    plan = hmi.batch_plan(sizes)
+    plan.set_autotune(autotune)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in sizes])[:-1]
+    x = fc.uniform_complex(plan.size_input(), 11)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    tx = torch.from_numpy(x).to(cuda).to(tdt)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    y = fc.to_host(ty)
+    for s, off, r in zip(sizes, offsets, oracle_each(oracle, sizes, offsets, x, -1, 0)):
+        assert fc.rel_l2(y[off:off + r.size], r) <= TOL[dtype], s
+    tz = torch.empty_like(tx)
+    plan.backward(ty, tz, hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(tz), x) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 def test_batch_all_smooth_lengths_and_offsets(hmi, cuda, oracle, dtype):
     """Every smooth length on every axis, output at shuffled, gapped offsets, symmetric scale."""
     import torch

@@ -480,7 +480,7 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
 // ------------------------------------------------------ staged compile-time tile
 
 template <typename RL, int T, int NTH, int S, typename C>
-__device__ __forceinline__ void staged_stages(const C *src, C *dst, const C *__restrict__ tw) {
+__device__ __forceinline__ void staged_stages(C *src, C *dst, const C *__restrict__ tw) {
     constexpr int Rr = RL::at(S), N = RL::n(), NB = N / Rr, Ls = RL::ls(S), TWO = RL::twoff(S);
     for (int idx = threadIdx.x; idx < T * NB; idx += NTH) {
         const int l = idx / NB, b = idx - l * NB;
@@ -488,7 +488,7 @@ __device__ __forceinline__ void staged_stages(const C *src, C *dst, const C *__r
         generic_bfly<Rr>(src + l * N + b, NB, dst + l * N + (b / Ls) * Ls * Rr + k, Ls, tw + TWO + k * (Rr - 1), k != 0);
     }
     __syncthreads();
-    if constexpr (S + 1 < RL::count) staged_stages<RL, T, NTH, S + 1>(dst, const_cast<C *>(src), tw);
+    if constexpr (S + 1 < RL::count) staged_stages<RL, T, NTH, S + 1>(dst, src, tw);
 }
 
 // One tile (T lines) of an n-point c2c pass of the batched kernel for a length whose radices

@@ -479,7 +479,7 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
 
 // ------------------------------------------------------ staged compile-time tile
 
-template <typename RL, int T, int NTH, int S, typename C>
+template <typename RL, int T, int NTH, int S, int END, typename C>
 __device__ __forceinline__ void staged_stages(C *src, C *dst, const C *__restrict__ tw) {
     constexpr int Rr = RL::at(S), N = RL::n(), NB = N / Rr, Ls = RL::ls(S), TWO = RL::twoff(S);
     for (int idx = threadIdx.x; idx < T * NB; idx += NTH) {
@@ -488,19 +488,24 @@ __device__ __forceinline__ void staged_stages(C *src, C *dst, const C *__restric
         generic_bfly<Rr>(src + l * N + b, NB, dst + l * N + (b / Ls) * Ls * Rr + k, Ls, tw + TWO + k * (Rr - 1), k != 0);
     }
     __syncthreads();
-    if constexpr (S + 1 < RL::count) staged_stages<RL, T, NTH, S + 1>(dst, src, tw);
+    if constexpr (S + 1 < END) staged_stages<RL, T, NTH, S + 1, END>(dst, src, tw);
 }
 
 // One tile (T lines) of an n-point c2c pass of the batched kernel for a length whose radices
 // have no small common multiple (105 = 3 * 5 * 7: a register tile would hold 105 elements per
 // thread).  Every stage runs one butterfly per work unit (T * n / radix of them) over the
-// NTH threads, from one LDS buffer to the other; radices, lines and strides are compile-time
-// constants and each line's offsets are computed once.  Same Stockham stages and twiddle
-// table as the register tiles.  LDS: 2 * T * n values + 2 * T offsets.
+// NTH threads; the first stage reads its inputs from global memory and the last stage writes
+// its outputs there (stage 0 has Ls = 1: outputs at b * r + q; the last stage has Ls = n / r:
+// outputs at b + q * n / r), the stages between go from one LDS buffer to the other.  Radices,
+// lines and strides are compile-time constants and each line's offsets are computed once.
+// Same Stockham stages and twiddle table as the register tiles.  LDS: 2 * T * n values + 2 * T
+// offsets.
 template <typename R, typename RL, int T, int NTH>
 __device__ __forceinline__ void staged_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *smem) {
     using C = typename cvec<R>::type;
     constexpr int N = RL::n();
+    constexpr int LAST = RL::count - 1;
+    static_assert(LAST >= 1, "a staged tile has at least two stages");
     C *buf0 = smem;
     C *buf1 = buf0 + T * N;
     long long *base = reinterpret_cast<long long *>(buf1 + T * N);  // [0, T) loads, [T, 2T) stores
@@ -513,26 +518,53 @@ __device__ __forceinline__ void staged_tile(const pass_args &a, unsigned tile, t
     }
     __syncthreads();
     const R sg = a.conj ? R(-1) : R(1);
-    const C *__restrict__ src = static_cast<const C *>(a.in);
-    for (int idx = tid; idx < T * N; idx += NTH) {
-        int l, j;
-        if (a.load_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
-        const long long b = base[l];
-        C x = b >= 0 ? src[b + (long long)j * a.in_sa] : cmk<C>(R(0), R(0));
-        buf0[l * N + j] = cmk<C>(x.x, sg * x.y);
+    const C *__restrict__ tw = static_cast<const C *>(a.tw);
+    {
+        // stage 0 from global memory into buf1
+        constexpr int R0 = RL::at(0), NB0 = N / R0;
+        const C *__restrict__ src = static_cast<const C *>(a.in);
+        for (int idx = tid; idx < T * NB0; idx += NTH) {
+            int l, b;
+            if (a.load_colfast) { l = idx % T; b = idx / T; } else { l = idx / NB0; b = idx % NB0; }
+            const long long lb = base[l];
+            C x[R0];
+#pragma unroll
+            for (int p = 0; p < R0; p++) {
+                const C v = lb >= 0 ? src[lb + (long long)(b + p * NB0) * a.in_sa] : cmk<C>(R(0), R(0));
+                x[p] = cmk<C>(v.x, sg * v.y);
+            }
+            dft<R0>::run(x);
+#pragma unroll
+            for (int q = 0; q < R0; q++) buf1[l * N + b * R0 + q] = x[q];
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    staged_stages<RL, T, NTH, 0>(buf0, buf1, static_cast<const C *>(a.tw));
-    const C *res = (RL::count % 2) ? buf1 : buf0;
-    const R sc = (R)a.scale;
-    C *__restrict__ dst = static_cast<C *>(a.out);
-    for (int idx = tid; idx < T * N; idx += NTH) {
-        int l, j;
-        if (a.store_colfast) { l = idx % T; j = idx / T; } else { l = idx / N; j = idx % N; }
-        const long long b = base[T + l];
-        if (b < 0) continue;
-        const C y = res[l * N + j];
-        dst[b + (long long)j * a.out_sa] = cmk<C>(y.x * sc, sg * y.y * sc);
+    if constexpr (LAST > 1) staged_stages<RL, T, NTH, 1, LAST>(buf1, buf0, tw);
+    {
+        // last stage from LDS (the output of stage LAST - 1) into global memory
+        constexpr int RR = RL::at(LAST), NB = N / RR, TWO = RL::twoff(LAST);
+        static_assert(RL::ls(LAST) == NB, "the last stage spans the whole line");
+        const C *res = ((LAST - 1) % 2 == 0) ? buf1 : buf0;
+        const R sc = (R)a.scale;
+        C *__restrict__ dst = static_cast<C *>(a.out);
+        for (int idx = tid; idx < T * NB; idx += NTH) {
+            int l, b;
+            if (a.store_colfast) { l = idx % T; b = idx / T; } else { l = idx / NB; b = idx % NB; }
+            const long long ob = base[T + l];
+            if (ob < 0) continue;
+            C x[RR];
+#pragma unroll
+            for (int p = 0; p < RR; p++) x[p] = res[l * N + b + p * NB];
+            if (b != 0) {
+                const C *w = tw + TWO + b * (RR - 1);
+#pragma unroll
+                for (int p = 1; p < RR; p++) x[p] = cmul(x[p], w[p - 1]);
+            }
+            dft<RR>::run(x);
+#pragma unroll
+            for (int q = 0; q < RR; q++)
+                dst[ob + (long long)(b + q * NB) * a.out_sa] = cmk<C>(x[q].x * sc, sg * x[q].y * sc);
+        }
     }
 }
 
@@ -629,7 +661,7 @@ __device__ __forceinline__ void batch_tile(int kid, const pass_args &a, const ge
 // One build per register class CLS (batch_kernel_class), held to kBatchWaves[CLS] waves per SIMD.
 // Splitting the tiles into three classes (from one build at 3 / 2 waves per SIMD) took the
 // 1000-transform batch from 11.4 to 6.7 ms per forward + backward pair; the staged tile of
-// length 105 (in place of the runtime-plan tile) to 5.9 ms.
+// length 105 (in place of the runtime-plan tile) to 5.7 ms.
 template <typename R, int CLS>
 __global__ void __launch_bounds__(sizeof(R) == 8 ? 128 : 256)
 __attribute__((amdgpu_waves_per_eu(kBatchWaves[CLS][sizeof(R) == 8])))

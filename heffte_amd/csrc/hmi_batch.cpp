@@ -396,14 +396,22 @@ batch_choice batch_tuner::choose(int n, bool dbl) {
     const std::vector<batch_choice> cands = candidates(n, dbl);
     batch_choice best = fallback(n, dbl);
     if (cands.size() > 1) {
-        double tbest = std::numeric_limits<double>::infinity();
+        // A runtime plan replaces the compile-time tile only when it is clearly faster: the
+        // probes are short, and a noisy timing (another process on the GPU, a profiler) must not
+        // lock a slower plan into the cache.
+        constexpr double kRuntimeMargin = 0.85;
+        const double inf = std::numeric_limits<double>::infinity();
+        double ttable = inf, trun = inf;
+        batch_choice run;
         for (const batch_choice &c : cands) {
             const double t = time_choice(n, dbl, c);
-            if (t < tbest) {
-                tbest = t;
-                best = c;
+            if (c.table) ttable = t;
+            else if (t < trun) {
+                trun = t;
+                run = c;
             }
         }
+        if (trun < inf && (ttable == inf || trun < kRuntimeMargin * ttable)) best = run;
     }
     pit->second[tune_key(n, dbl)] = best;
     save_cache(path, pit->second);

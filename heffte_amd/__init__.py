@@ -57,6 +57,7 @@ def _load():
     lib.heffte_mi355_get_kind.argtypes = [plan_p]
     lib.heffte_mi355_set_stream.argtypes = [plan_p, vp]
     lib.heffte_mi355_set_precision.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_plan_prepare.argtypes = [plan_p, ctypes.c_int]
     for f in ("forward_c2c", "forward_z2z", "forward_s2c", "forward_d2z", "backward_c2c", "backward_z2z",
               "backward_c2s", "backward_z2d", "forward_s2s", "forward_d2d", "backward_s2s", "backward_d2d"):
         fn = getattr(lib, "heffte_mi355_" + f)
@@ -74,7 +75,7 @@ def _load():
     lib.heffte_mi355_batch_set_stream.argtypes = [plan_p, vp]
     lib.heffte_mi355_batch_set_graph.argtypes = [plan_p, ctypes.c_int]
     lib.heffte_mi355_batch_set_autotune.argtypes = [plan_p, ctypes.c_int]
-    for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output"):
+    for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output", "heffte_mi355_batch_graph_captures"):
         getattr(lib, f).argtypes = [plan_p]
         getattr(lib, f).restype = ctypes.c_longlong
     for f in ("forward_c2c", "forward_z2z", "backward_c2c", "backward_z2z"):
@@ -102,7 +103,8 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_forward_c2c", "heffte_mi355_batch_forward_z2z", "heffte_mi355_batch_backward_c2c",
     "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
     "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
-    "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z",
+    "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z", "heffte_mi355_plan_prepare",
+    "heffte_mi355_batch_graph_captures",
 )
 
 
@@ -217,6 +219,11 @@ class fft_plan:
     def set_precision(self, form):
         """precision.fp32 (default), precision.split_fp16 or precision.split_bf16."""
         _check(lib.heffte_mi355_set_precision(self._h, int(form)), "set_precision")
+
+    def prepare(self, double_precision=False):
+        """Uploads the device tables of every transform variant in one precision now, so that
+        later calls (e.g. under stream capture) upload nothing."""
+        _check(lib.heffte_mi355_plan_prepare(self._h, 1 if double_precision else 0), "prepare")
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream, a raw hipStream_t address, or None (default stream)."""
@@ -420,6 +427,11 @@ class batch_plan:
 
     def __init__(self, sizes, in_offsets=None, out_offsets=None):
         sizes = [tuple(int(v) for v in s) for s in sizes]
+        if any(len(s) != 3 for s in sizes):
+            raise heffte_input_error("every transform of a batch has three sides")
+        for name, offs in (("in_offsets", in_offsets), ("out_offsets", out_offsets)):
+            if offs is not None and len(offs) != len(sizes):
+                raise heffte_input_error(f"{name} holds {len(offs)} offsets for {len(sizes)} transforms")
         flat = [v for s in sizes for v in s]
         S = (ctypes.c_int * len(flat))(*flat)
         io = (ctypes.c_longlong * len(sizes))(*in_offsets) if in_offsets is not None else None
@@ -447,6 +459,10 @@ class batch_plan:
 
     def set_graph(self, enable):
         _check(lib.heffte_mi355_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
+
+    def graph_captures(self):
+        """Number of hipGraph captures made so far (a replayed call adds none)."""
+        return lib.heffte_mi355_batch_graph_captures(self._h)
 
     def set_autotune(self, enable):
         """Time the tile candidates of each side length on the first call (default) or not."""

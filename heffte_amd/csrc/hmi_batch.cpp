@@ -60,6 +60,20 @@ batch_plan::batch_plan(int count, const int *sizes, const long long *in_off, con
         out_pack += cnt;
         in_extent_ = std::max(in_extent_, x.in_off + cnt);
         out_extent_ = std::max(out_extent_, x.out_off + cnt);
+        in_first_ = i == 0 ? x.in_off : std::min(in_first_, x.in_off);
+        out_first_ = i == 0 ? x.out_off : std::min(out_first_, x.out_off);
+        same_offsets_ = same_offsets_ && x.in_off == x.out_off;
+    }
+    // every phase writes each transform's output range while the others are read and written:
+    // output ranges must be disjoint
+    {
+        std::vector<std::pair<long long, long long>> r(count);
+        for (int i = 0; i < count; i++)
+            r[i] = {xf_[i].out_off, xf_[i].out_off + (long long)xf_[i].n[0] * xf_[i].n[1] * xf_[i].n[2]};
+        std::sort(r.begin(), r.end());
+        for (int i = 1; i < count; i++)
+            if (r[i].first < r[i - 1].second)
+                throw std::invalid_argument("the output ranges of two transforms of the batch overlap");
     }
     for (int n : lengths_)
         if (2LL * n * 16 > kGenericMaxLds)
@@ -67,8 +81,11 @@ batch_plan::batch_plan(int count, const int *sizes, const long long *in_off, con
 }
 
 batch_plan::~batch_plan() {
-    for (auto &g : graph_)
-        if (g) (void)hipGraphExecDestroy(g);
+    bool any = false;
+    for (auto &m : graphs_) any = any || !m.empty();
+    if (any) (void)hipStreamSynchronize(stream_);  // a replay may still run
+    for (auto &m : graphs_)
+        for (auto &kv : m) (void)hipGraphExecDestroy(kv.second);
     if (capture_stream_) (void)hipStreamDestroy(capture_stream_);
     for (auto &d : dev_) {
         (void)hipFree(d.tw);
@@ -172,7 +189,9 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
             // persistent grid of 2048 blocks walking the list was 13 % slower on the
             // 1000-transform batch); the kernel walks the list with a grid stride, so any grid
             // size is valid
-            const int nblocks = b.nitems;
+            // at most 2^32 - 1 work-items in a grid: beyond that the grid stride covers the rest
+            const int max_blocks = (int)(0xffffffffLL / (dbl ? 128 : 256));
+            const int nblocks = std::min(b.nitems, max_blocks);
             if (nblocks > 0) check_hip(launch_batched(dbl, c, b, nblocks, d.lds[c], s), "batched FFT phase");
         }
     }
@@ -180,6 +199,21 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
 
 void batch_plan::execute(int dir, bool dbl, const void *in, void *out, int scaling) {
     if (!in || !out) throw std::invalid_argument("null input or output array");
+    {
+        // aliasing: phase 0 reads every input range while it writes every output range
+        const size_t elem = dbl ? 16 : 8;
+        if (in == out) {
+            if (!same_offsets_)
+                throw std::invalid_argument("in-place batch calls need equal input and output offsets per transform");
+        } else {
+            const uintptr_t i0 = reinterpret_cast<uintptr_t>(in) + (uintptr_t)in_first_ * elem;
+            const uintptr_t i1 = reinterpret_cast<uintptr_t>(in) + (uintptr_t)in_extent_ * elem;
+            const uintptr_t o0 = reinterpret_cast<uintptr_t>(out) + (uintptr_t)out_first_ * elem;
+            const uintptr_t o1 = reinterpret_cast<uintptr_t>(out) + (uintptr_t)out_extent_ * elem;
+            if (i0 < o1 && o0 < i1)
+                throw std::invalid_argument("the input and output ranges of the batch overlap (and in != out)");
+        }
+    }
     std::lock_guard<std::mutex> lock(mtx_);
     prepare(dbl);
     if (!use_graph_) {
@@ -188,11 +222,14 @@ void batch_plan::execute(int dir, bool dbl, const void *in, void *out, int scali
     }
     const int g = dbl ? 1 : 0;
     graph_key key{dir, scaling, in, const_cast<void *>(out), stream_};
-    if (!graph_[g] || !(gkey_[g] == key)) {
-        // capture the three phase launches once per (direction, scale, arrays) and replay them
-        if (graph_[g]) {
-            check_hip(hipGraphExecDestroy(graph_[g]), "hipGraphExecDestroy");
-            graph_[g] = nullptr;
+    auto found = graphs_[g].find(key);
+    if (found == graphs_[g].end()) {
+        // capture the three phase launches once per (direction, scale, arrays, stream) and replay
+        // them; a full cache is emptied first, after the replays in flight have finished
+        if (graphs_[g].size() >= kMaxGraphs) {
+            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+            for (auto &kv : graphs_[g]) (void)hipGraphExecDestroy(kv.second);
+            graphs_[g].clear();
         }
         if (!capture_stream_) check_hip(hipStreamCreateWithFlags(&capture_stream_, hipStreamNonBlocking), "hipStreamCreate");
         hipGraph_t graph = nullptr;
@@ -210,10 +247,10 @@ void batch_plan::execute(int dir, bool dbl, const void *in, void *out, int scali
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         check_hip(e, "hipGraphInstantiate");
-        graph_[g] = exec;
-        gkey_[g] = key;
+        found = graphs_[g].emplace(key, exec).first;
+        captures_++;
     }
-    check_hip(hipGraphLaunch(graph_[g], stream_), "hipGraphLaunch");
+    check_hip(hipGraphLaunch(found->second, stream_), "hipGraphLaunch");
 }
 
 
@@ -296,17 +333,23 @@ void save_cache(const std::string &path, const std::map<long long, batch_choice>
 
 }  // namespace
 
-// Fastest of five runs of a probe batch (one n x n x m transform, m a power of two with
-// about 2^20 elements in all), forward, in place, on a stream of its own.
+// Fastest of five runs of a probe batch, forward, in place, on a stream of its own: one
+// n x n2 x m transform of at most about 2^20 elements (n2 = n and m a power of two from 16 to 512
+// for n <= 256; n2 = max(1, 2^16 / n) and m = 16 for longer sides), so the probe stays at most
+// 16 MiB whatever the side.
 double batch_tuner::time_choice(int n, bool dbl, const batch_choice &c) {
-    long long m = 16;
-    while (m < 512 && (long long)n * n * m * 2 <= (1LL << 20)) m *= 2;
-    const int sizes[3] = {n, n, (int)m};
+    long long m = 16, n2 = n;
+    if (n <= 256) {
+        while (m < 512 && (long long)n * n * m * 2 <= (1LL << 20)) m *= 2;
+    } else {
+        n2 = std::max(1, (1 << 16) / n);
+    }
+    const int sizes[3] = {n, (int)n2, (int)m};
     batch_plan probe(1, sizes, nullptr, nullptr);
     probe.set_graph(false);
     probe.set_autotune(false);
     probe.forced_[n] = c;
-    const size_t bytes = (size_t)n * n * m * (dbl ? 16 : 8);
+    const size_t bytes = (size_t)n * n2 * m * (dbl ? 16 : 8);
     void *buf = nullptr;
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -404,7 +447,14 @@ batch_choice batch_tuner::choose(int n, bool dbl) {
         double ttable = inf, trun = inf;
         batch_choice run;
         for (const batch_choice &c : cands) {
-            const double t = time_choice(n, dbl, c);
+            double t = inf;
+            try {
+                t = time_choice(n, dbl, c);
+            } catch (const std::runtime_error &) {
+                // the probe could not run (device memory): keep the untimed default
+                (void)hipGetLastError();
+                return best;
+            }
             if (c.table) ttable = t;
             else if (t < trun) {
                 trun = t;

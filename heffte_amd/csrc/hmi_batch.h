@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <map>
 #include <mutex>
 #include <string>
@@ -50,8 +51,12 @@ class batch_plan {
     int count() const { return (int)xf_.size(); }
     long long input_elems() const { return in_extent_; }
     long long output_elems() const { return out_extent_; }
-    // dir: -1 forward, +1 backward
+    // dir: -1 forward, +1 backward.  Aliasing rule: in == out only when every transform's input
+    // and output offsets are equal (each transform in place on its own range); otherwise the
+    // input and output element ranges must not overlap.  Violations throw std::invalid_argument.
     void execute(int dir, bool dbl, const void *in, void *out, int scaling);
+    // number of hipGraph captures so far (a replayed call captures nothing)
+    long long graph_captures() const { return captures_; }
 
   private:
     struct device_state {
@@ -72,24 +77,33 @@ class batch_plan {
         const void *in;
         void *out;
         hipStream_t stream;
-        bool operator==(const graph_key &o) const {
-            return dir == o.dir && scaling == o.scaling && in == o.in && out == o.out && stream == o.stream;
+        bool operator<(const graph_key &o) const {
+            if (dir != o.dir) return dir < o.dir;
+            if (scaling != o.scaling) return scaling < o.scaling;
+            if (in != o.in) return std::less<const void *>()(in, o.in);
+            if (out != o.out) return std::less<void *>()(out, o.out);
+            return std::less<hipStream_t>()(stream, o.stream);
         }
     };
+    // instantiated graphs kept per precision; beyond this many the stream is synchronised and
+    // the cache is emptied before a new capture
+    static constexpr size_t kMaxGraphs = 16;
     void prepare(bool dbl);
     void launch_phases(int dir, bool dbl, const void *in, void *out, int scaling, hipStream_t s);
 
     std::vector<batch_xform> xf_;
     std::vector<int> lengths_;  // distinct axis lengths, index = size_id
     long long in_extent_ = 0, out_extent_ = 0;
+    long long in_first_ = 0, out_first_ = 0;  // smallest input / output offset
+    bool same_offsets_ = true;                // in_off == out_off for every transform
     hipStream_t stream_ = nullptr;
     bool use_graph_ = true;
     bool autotune_ = true;
     std::map<int, batch_choice> forced_;  // probe plans of the tuner
     friend class batch_tuner;
     device_state dev_[2];
-    hipGraphExec_t graph_[2] = {nullptr, nullptr};
-    graph_key gkey_[2] = {};
+    std::map<graph_key, hipGraphExec_t> graphs_[2];
+    long long captures_ = 0;
     hipStream_t capture_stream_ = nullptr;
     std::mutex mtx_;
 };

@@ -181,6 +181,11 @@ int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision) {
     return HMI_SUCCESS;
 }
 
+int heffte_mi355_plan_prepare(heffte_mi355_plan plan, int double_precision) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    return guarded([&] { plan->impl->prepare(double_precision != 0); });
+}
+
 long long heffte_mi355_size_convolve_workspace(heffte_mi355_plan plan) {
     return valid(plan) && plan->impl->get_kind() == hmi::kind::c2c ? plan->impl->convolve_workspace() : -1;
 }
@@ -289,6 +294,10 @@ long long heffte_mi355_batch_size_input(heffte_mi355_batch batch) {
 }
 long long heffte_mi355_batch_size_output(heffte_mi355_batch batch) {
     return valid_batch(batch) ? batch->impl->output_elems() : -1;
+}
+
+long long heffte_mi355_batch_graph_captures(heffte_mi355_batch batch) {
+    return valid_batch(batch) ? batch->impl->graph_captures() : -1;
 }
 
 static int batch_run(heffte_mi355_batch batch, int dir, bool dbl, void const *in, void *out, int scale) {

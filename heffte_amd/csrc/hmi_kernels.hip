@@ -45,6 +45,10 @@ template <int... Rs> struct radices {
 
 template <int PS> __device__ __forceinline__ int padj(int j) { return j + (j >> PS); }
 
+// Row stride (complex elements of `bytes` bytes) of the LDS tile of an n-point line with one pad
+// element per 2^ps, for tiles of t lines.
+constexpr int tile_row_stride(int n, int ps, int t, int bytes) { return n + (n >> ps) + 1 + 0 * t * bytes; }
+
 template <int S, typename RL, int E, int P, int RS, int PS, typename C>
 __device__ __forceinline__ void run_stages(C (&v)[E], C *lds, const C *__restrict__ tw, int l, int t, int l2,
                                            int t2) {
@@ -101,7 +105,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     using C = typename cvec<R>::type;
     constexpr int N = RL::n();
     constexpr int P = N / E;
-    constexpr int RS = N + (N >> PS) + 1;
+    constexpr int RS = tile_row_stride(N, PS, T, sizeof(C));
 
     const int tid = threadIdx.x;
     int l, t, l2, t2;
@@ -340,7 +344,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
 template <typename R, typename RL, int E, int T, int PS, bool NT>
 __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     using C = typename cvec<R>::type;
-    constexpr int RS = RL::n() + (RL::n() >> PS) + 1;
+    constexpr int RS = tile_row_stride(RL::n(), PS, T, sizeof(C));
     __shared__ C lds[T * RS];
     uniform_tile<R, RL, E, T, PS, NT>(a, blockIdx.x, lds);
 }
@@ -604,7 +608,7 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
     X(16, 105, 0, 16, 0, 3, 5, 7)
 
 constexpr size_t batch_lds_bytes(int n, int e, int t, int ps, size_t value) {
-    return e == 0 ? (size_t)2 * t * n * value + (size_t)2 * t * 8 : (size_t)(t + 1) * (n + (n >> ps) + 1) * value;
+    return e == 0 ? (size_t)2 * t * n * value + (size_t)2 * t * 8 : (size_t)(t + 1) * tile_row_stride(n, ps, t, (int)value) * value;
 }
 #define HMI_BATCH_INFO(K, N, E, T, PS, ...) {N, T, T / 2, {__VA_ARGS__}, radices<__VA_ARGS__>::count, \
     batch_lds_bytes(N, E, T, PS, 8), batch_lds_bytes(N, E, T / 2, PS, 16)},
@@ -847,14 +851,15 @@ const fast_kernel_info *find_fast_kernel(int n, bool dbl, bool wide) {
 
 hipError_t launch_generic(bool dbl, const pass_args &a, const generic_plan &g, long long nblocks, size_t lds_bytes,
                           hipStream_t s) {
-    static bool configured = false;
-    if (!configured) {
+    // once per process, thread-safe (static initialisation)
+    static const bool configured = [] {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_generic<float>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_generic<double>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
-        configured = true;
-    }
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        return true;
+    }();
+    (void)configured;
     if (dbl) fft_generic<double><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(a, g);
     else fft_generic<float><<<dim3((unsigned)nblocks), dim3(256), lds_bytes, s>>>(a, g);
     return hipGetLastError();

@@ -90,6 +90,13 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
     }
     // workspace: the largest over every variant the plan may run
     long long ws = 0;
+    for_each_variant([&](int dir, bool ir, bool orl, bool ip) { ws = std::max(ws, build(dir, false, ir, orl, ip).ws_elems); });
+    ws_elems_ = ws;
+}
+
+// Calls f(dir, in_real, out_real, in_place) for every overload of fft3d the plan's kind has.
+template <typename F> void fft_plan::for_each_variant(F &&f) const {
+    const kind k = kind_;
     for (int dir : {-1, 1})
         for (int ir = 0; ir < 2; ir++)
             for (int orl = 0; orl < 2; orl++)
@@ -98,10 +105,19 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
                     if (k == kind::c2c && orl && dir == -1) continue;
                     if (k == kind::r2c && (ir != (dir == -1) || orl != (dir == 1))) continue;
                     if ((k == kind::cos || k == kind::sin || k == kind::cos1) && (!ir || !orl)) continue;
-                    exec_plan p = build(dir, false, ir, orl, ip);
-                    ws = std::max(ws, p.ws_elems);
+                    f(dir, ir != 0, orl != 0, ip != 0);
                 }
-    ws_elems_ = ws;
+}
+
+void fft_plan::prepare(bool dbl) {
+    for_each_variant([&](int dir, bool ir, bool orl, bool ip) {
+        const exec_plan &ep = get_exec_plan(dir, dbl, ir, orl, ip);
+        for (const pass &p : ep.passes) (void)get_twiddles(dbl, p, true);
+    });
+    if (!dbl && precision_ != kPrecPlain && kind_ == kind::c2c) {
+        (void)get_mfma_table(precision_, false, true);
+        (void)get_mfma_table(precision_, true, true);
+    }
 }
 
 fft_plan::~fft_plan() {
@@ -447,13 +463,30 @@ std::vector<double> stage_table_f64(const int *radix, int nstages) {
     return make_table<double>(radix, nstages, false, 0, 1);
 }
 
-const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
+namespace {
+// device copy of a host table (freed again if the copy fails)
+void *upload_bytes(const void *src, size_t bytes) {
+    void *d = nullptr;
+    check_hip(hipMalloc(&d, bytes), "hipMalloc(twiddles)");
+    const hipError_t e = hipMemcpy(d, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) (void)hipFree(d);
+    check_hip(e, "hipMemcpy(twiddles)");
+    return d;
+}
+
+const char *kNotResident =
+    "device tables are not resident: a call that may not allocate (stream capture, or the no-allocation entry "
+    "point) needs heffte_mi355_plan_prepare on this plan and precision first";
+}
+
+const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p, bool may_alloc) {
     const int n = p.n;
     const long long key = ((long long)n << 8) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
                           ((long long)p.mode << 3) | (p.fast ? (long long)p.fast->T << 40 : 0);
     std::lock_guard<std::mutex> lock(mtx_);
     auto it = tw_.find(key);
     if (it != tw_.end()) return it->second;
+    if (!may_alloc) throw std::runtime_error(kNotResident);
     const int *radix = p.fast ? p.fast->radix : p.gen.radix;
     const int ns = p.fast ? p.fast->nstages : p.gen.nstages;
     twiddles t;
@@ -465,23 +498,52 @@ const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p) {
     if (dbl) {
         std::vector<double> v = make_table<double>(radix, ns, !p.fast, tail_count, tail_mod);
         t.bytes = v.size() * sizeof(double);
-        check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
-        check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+        t.dev = upload_bytes(v.data(), t.bytes);
     } else {
         std::vector<float> v = make_table<float>(radix, ns, !p.fast, tail_count, tail_mod);
         t.bytes = v.size() * sizeof(float);
-        check_hip(hipMalloc(&t.dev, t.bytes), "hipMalloc(twiddles)");
-        check_hip(hipMemcpy(t.dev, v.data(), t.bytes, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+        t.dev = upload_bytes(v.data(), t.bytes);
     }
     return tw_.emplace(key, t).first->second;
 }
 
+const twiddles &fft_plan::get_mfma_table(int prec, bool conj, bool may_alloc) {
+    // W512 and the split W32 / W16 operand tables of one precision form and direction
+    std::lock_guard<std::mutex> lock(mtx_);
+    twiddles &mt = mfma_tw_[prec == kSplitF16 ? 0 : 1][conj ? 1 : 0];
+    if (!mt.dev) {
+        if (!may_alloc) throw std::runtime_error(kNotResident);
+        std::vector<unsigned char> t = mfma512_operand_table(prec, conj);
+        mt.dev = upload_bytes(t.data(), t.size());
+        mt.bytes = t.size();
+    }
+    return mt;
+}
+
 void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
-                       int scaling, const void *mul) {
+                       int scaling, const void *mul, bool no_alloc) {
     if (batch < 1) throw std::invalid_argument("batch size must be positive");
     if (!in || !out) throw std::invalid_argument("null input or output array");
     const bool in_place = (in == out);
     const exec_plan &ep = get_exec_plan(dir, dbl, in_real, out_real, in_place);
+    // the device tables must be resident before anything is launched when the call may not
+    // allocate: look them all up first (a missing one throws here, with nothing enqueued)
+    bool may_alloc = !no_alloc;
+    if (may_alloc) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream_, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) may_alloc = false;
+    }
+    const bool split = precision_ != kPrecPlain && !dbl;
+    constexpr size_t kMaxPasses = 8;
+    if (ep.passes.size() > kMaxPasses) throw std::runtime_error("internal error: too many passes");
+    std::array<const void *, kMaxPasses> tables{};
+    for (size_t i = 0; i < ep.passes.size(); i++) {
+        const pass &p = ep.passes[i];
+        const bool mfma = split && p.n == 512 && p.mode == kModeC2C && !p.identity && !p.in_real && !p.out_real &&
+                          !p.herm_load && p.store_len == p.n && !(mul && i == 0);
+        tables[i] = mfma ? get_mfma_table(precision_, dir > 0, may_alloc).dev : get_twiddles(dbl, p, may_alloc).dev;
+    }
+    if (no_alloc && ep.ws_elems > 0 && !ws) throw std::invalid_argument("a call that may not allocate needs a workspace");
     void *tmp = nullptr;
     if (ep.ws_elems > 0 && !ws) {
         // fft3d::forward(in, out) allocates the workspace per call (include/heffte_fft3d.h:353-360)
@@ -518,8 +580,7 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         };
         a.in = bufptr(p.src);
         a.out = bufptr(p.dst);
-        const twiddles &tw = get_twiddles(dbl, p);
-        a.tw = tw.dev;
+        a.tw = tables[&p - &ep.passes.front()];
         a.in_sa = p.lin.stride[p.axis];
         a.in_sc = p.lin.stride[p.col];
         a.in_sb1 = p.lin.stride[p.b1];
@@ -556,24 +617,12 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
             a.mul = mul;
         }
         const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
-        const bool mfma = precision_ != kPrecPlain && !dbl && p.n == 512 && p.mode == kModeC2C && !p.identity &&
-                          !p.in_real && !p.out_real && !p.herm_load && p.store_len == p.n && !a.mul;
+        const bool mfma = split && p.n == 512 && p.mode == kModeC2C && !p.identity && !p.in_real && !p.out_real &&
+                          !p.herm_load && p.store_len == p.n && !a.mul;
         if (mfma) {
-            // the split-precision form: W512 and the split W32 / W16 operand tables, 16-line
-            // tiles; the backward pass multiplies by the conjugate matrices directly (no
-            // conjugation of the data on load and store)
-            const int pi = precision_ == kSplitF16 ? 0 : 1, di = dir > 0 ? 1 : 0;
-            {
-                std::lock_guard<std::mutex> lock(mtx_);
-                twiddles &mt = mfma_tw_[pi][di];
-                if (!mt.dev) {
-                    std::vector<unsigned char> t = mfma512_operand_table(precision_, di == 1);
-                    mt.bytes = t.size();
-                    check_hip(hipMalloc(&mt.dev, t.size()), "hipMalloc(twiddles)");
-                    check_hip(hipMemcpy(mt.dev, t.data(), t.size(), hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
-                }
-                a.tw = mt.dev;
-            }
+            // the split-precision form: a.tw holds W512 and the split W32 / W16 operand tables
+            // of the direction, 16-line tiles; the backward pass multiplies by the conjugate
+            // matrices directly (no conjugation of the data on load and store)
             a.conj = 0;
             check_hip(launch_mfma512(precision_, a, ((long long)a.nlines + 15) / 16, stream_), "MFMA FFT pass");
         } else if (p.fast) {

@@ -96,8 +96,15 @@ class fft_plan {
     // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
     // overloads of fft3d (real input on forward, real output on backward).  mul (c2c only):
     // the first pass loads in[i] * mul[i] instead of in[i] (mul laid out like in).
+    // no_alloc: the call allocates nothing -- ws must be given and the device tables must be
+    // resident (prepare); otherwise the call throws before it launches anything.  A call on a
+    // stream under capture is held to the same rule for the tables (their upload is a
+    // synchronous copy, which a capture cannot record).
     void execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
-                 int scaling, const void *mul = nullptr);
+                 int scaling, const void *mul = nullptr, bool no_alloc = false);
+    // Uploads the device tables (twiddles, split-precision operands of the current precision)
+    // of every variant of the plan in one precision, so that later calls upload nothing.
+    void prepare(bool dbl);
     // out = backward(forward(x) * forward(y)) with the backward scaling (scale::full gives the
     // circular convolution of x and y); the product is fused into the first backward pass,
     // and x == y runs one forward transform.
@@ -115,7 +122,9 @@ class fft_plan {
   private:
     exec_plan build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const;
     void configure(pass &p, bool dbl) const;
-    const twiddles &get_twiddles(bool dbl, const pass &p);
+    const twiddles &get_twiddles(bool dbl, const pass &p, bool may_alloc);
+    const twiddles &get_mfma_table(int prec, bool conj, bool may_alloc);
+    template <typename F> void for_each_variant(F &&f) const;
 
     kind kind_;
     box3 inbox_, outbox_;

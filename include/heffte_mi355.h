@@ -112,6 +112,15 @@ int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream);
 #define HMI_PRECISION_SPLIT_BF16 2
 int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision);
 
+/* Uploads the plan's device tables (twiddles, and the split-precision operands of the current
+ * precision form) for every transform direction and overload, in single (double_precision = 0)
+ * or double precision.  No heFFTe counterpart: heFFTe's rocFFT executors create their plans
+ * lazily on the first call (include/heffte_backend_rocm.h:705-715).  Without it the tables are
+ * uploaded on a plan's first call; a first call on a stream under capture, which cannot
+ * record that synchronous upload, fails with HMI_ERROR_DEVICE and enqueues nothing.  Call it
+ * again after heffte_mi355_set_precision selects a split-precision form. */
+int heffte_mi355_plan_prepare(heffte_mi355_plan plan, int double_precision);
+
 /* Forward transforms; replace heffte_forward_{s2c,c2c,d2z,z2z}_buffered
  * (include/heffte_c.h:164-179) with a batch size.  workspace may be NULL, in
  * which case it is allocated for the call (fft3d::forward(in, out),
@@ -188,7 +197,12 @@ int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const
  * out_offsets NULL = same as in_offsets).  The forward/backward calls run three phases
  * (one per axis; each one launch per register class of tile) over device work lists, replayed
  * from a captured hipGraph unless
- * heffte_mi355_batch_set_graph(batch, 0) was called.  scale applies per transform (1/N_i). */
+ * heffte_mi355_batch_set_graph(batch, 0) was called; one instantiated graph is kept per
+ * (direction, scale, input, output, stream), so calls that alternate between a few argument
+ * sets replay without capturing again.  scale applies per transform (1/N_i).
+ * Aliasing: the output ranges of the transforms must be disjoint (checked at creation);
+ * input == output only when in_offsets[i] == out_offsets[i] for every i, otherwise the input and
+ * output ranges must not overlap (checked per call, HMI_ERROR_INVALID_ARGUMENT). */
 typedef struct heffte_mi355_batch_s *heffte_mi355_batch;
 
 int heffte_mi355_batch_create(int count, int const *sizes, long long const *in_offsets,
@@ -211,6 +225,8 @@ int heffte_mi355_batch_forward_c2c(heffte_mi355_batch batch, void const *input, 
 int heffte_mi355_batch_forward_z2z(heffte_mi355_batch batch, void const *input, void *output, int scale);
 int heffte_mi355_batch_backward_c2c(heffte_mi355_batch batch, void const *input, void *output, int scale);
 int heffte_mi355_batch_backward_z2z(heffte_mi355_batch batch, void const *input, void *output, int scale);
+/* Number of hipGraph captures the batch has made (diagnostic: a replayed call adds none). */
+long long heffte_mi355_batch_graph_captures(heffte_mi355_batch batch);
 
 /* Text of the last error raised on the calling thread ("" if none). */
 char const *heffte_mi355_last_error(void);

@@ -182,3 +182,31 @@ def test_batch_c_abi_rejects_bad_handles(hmi):
     h = ctypes.c_void_p()
     assert lib.heffte_mi355_batch_create(1, None, None, None, ctypes.byref(h)) != 0
     assert "size" in hmi.last_error()
+
+
+def test_batch_aliasing_rules(hmi):
+    """Output ranges of a batch must be disjoint (checked at creation); a call with input ==
+    output needs equal offsets per transform, and otherwise the input and output ranges must
+    not overlap (checked before anything is launched, so these run without a GPU)."""
+    lib = hmi.lib
+    with pytest.raises(hmi.heffte_input_error) as e:
+        hmi.batch_plan([(4, 4, 4), (2, 2, 2)], in_offsets=[0, 64], out_offsets=[0, 60])
+    assert "overlap" in str(e.value)
+    with pytest.raises(hmi.heffte_input_error) as e:
+        hmi.batch_plan([(4, 4, 4), (2, 2, 2)], in_offsets=[0])  # shorter than the list of sizes
+    assert "offsets" in str(e.value)
+    # in == out with different offsets per transform: rejected
+    p = hmi.batch_plan([(4, 4, 4), (2, 2, 2)], in_offsets=[8, 72], out_offsets=[0, 64])
+    base = 1 << 40  # a fake device address: the check fails before any GPU call
+    assert lib.heffte_mi355_batch_forward_c2c(p._h, ctypes.c_void_p(base), ctypes.c_void_p(base), 0) == 2
+    assert "equal input and output offsets" in hmi.last_error()
+    # distinct arrays whose element ranges overlap: rejected
+    assert lib.heffte_mi355_batch_forward_c2c(p._h, ctypes.c_void_p(base), ctypes.c_void_p(base + 8 * 16), 0) == 2
+    assert "overlap" in hmi.last_error()
+    assert lib.heffte_mi355_batch_backward_z2z(p._h, ctypes.c_void_p(base + 16 * 40), ctypes.c_void_p(base), 0) == 2
+    assert lib.heffte_mi355_batch_graph_captures(p._h) == 0
+    assert lib.heffte_mi355_batch_graph_captures(None) == -1
+
+
+def test_prepare_rejects_bad_handles(hmi):
+    assert hmi.lib.heffte_mi355_plan_prepare(None, 0) == 3

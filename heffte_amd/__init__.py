@@ -63,6 +63,9 @@ def _load():
         fn = getattr(lib, "heffte_mi355_" + f)
         fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, ctypes.c_int]
         fn.restype = ctypes.c_int
+    lib.heffte_mi355_execute_c2c_noalloc.argtypes = [plan_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
+                                                     ctypes.c_int]
+    lib.heffte_mi355_execute_c2c_noalloc.restype = ctypes.c_int
     lib.heffte_mi355_reshape.argtypes = [ctypes.c_int, LL3, LL3, I3, I3, ctypes.c_int, vp, vp, vp]
     lib.heffte_mi355_size_convolve_workspace.argtypes = [plan_p]
     lib.heffte_mi355_size_convolve_workspace.restype = ctypes.c_longlong
@@ -104,7 +107,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
     "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
     "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z", "heffte_mi355_plan_prepare",
-    "heffte_mi355_batch_graph_captures",
+    "heffte_mi355_batch_graph_captures", "heffte_mi355_execute_c2c_noalloc",
 )
 
 

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libheffte_mi355.so")
 ARCH = os.environ.get("HMI_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hmi_kernels.hip", "hmi_mfma.hip", "hmi_plan.cpp", "hmi_batch.cpp", "hmi_capi.cpp"]
+SOURCES = ["hmi_kernels.hip", "hmi_mfma.hip", "hmi_scratch.hip", "hmi_plan.cpp", "hmi_batch.cpp", "hmi_capi.cpp"]
 
 
 def hipcc():

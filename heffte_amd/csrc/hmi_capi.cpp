@@ -241,6 +241,19 @@ int heffte_mi355_backward_d2d(heffte_mi355_plan p, int b, double const *in, doub
     return run_checked(p, true, 1, true, true, true, b, in, out, ws, s);
 }
 
+int heffte_mi355_execute_c2c_noalloc(heffte_mi355_plan plan, int direction, int double_precision, int batch,
+                                     void const *input, void *output, int scale) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    if (plan->impl->get_kind() != hmi::kind::c2c)
+        return fail(HMI_ERROR_INVALID_ARGUMENT, "the no-allocation call takes a c2c plan");
+    if (direction != -1 && direction != 1) return fail(HMI_ERROR_INVALID_ARGUMENT, "direction must be -1 or +1");
+    if (scale < 0 || scale > 2) return fail(HMI_ERROR_INVALID_ARGUMENT, "scale must be 0, 1 or 2");
+    return guarded([&] {
+        plan->impl->execute(direction, double_precision != 0, false, false, batch, input, output, nullptr, scale,
+                            nullptr, true);
+    });
+}
+
 int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const high[3], int const in_order[3],
                          int const out_order[3], int batch, void const *input, void *output, void *hip_stream) {
     return guarded([&] {

@@ -77,4 +77,10 @@ struct permute_args {
 
 hipError_t launch_permute(int elem_bytes, const void *src, void *dst, const permute_args &p, hipStream_t s);
 
+// Static device workspace of the no-allocation entry point (hmi_scratch.hip): kScratchBytes per
+// GPU, held until the process ends.  scratch_address() returns its device address on the current
+// device (resolved with hipGetSymbolAddress on the first call per device; throws on failure).
+constexpr size_t kScratchBytes = 64ull << 20;
+void *scratch_address();
+
 }  // namespace hmi

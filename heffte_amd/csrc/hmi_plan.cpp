@@ -526,6 +526,16 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
     if (!in || !out) throw std::invalid_argument("null input or output array");
     const bool in_place = (in == out);
     const exec_plan &ep = get_exec_plan(dir, dbl, in_real, out_real, in_place);
+    if (no_alloc) {
+        // the workspace is the static scratch of this device (one per GPU, shared by every call)
+        if (ws) throw std::invalid_argument("the no-allocation call takes no workspace (it uses the static scratch)");
+        const unsigned long long need = (unsigned long long)ep.ws_elems * (unsigned long long)batch * (dbl ? 16 : 8);
+        if (need > kScratchBytes)
+            throw std::invalid_argument("the workspace of this call (" + std::to_string(need) +
+                                        " bytes) exceeds the static scratch of the no-allocation entry point (" +
+                                        std::to_string(kScratchBytes) + " bytes)");
+        if (need > 0) ws = scratch_address();
+    }
     // the device tables must be resident before anything is launched when the call may not
     // allocate: look them all up first (a missing one throws here, with nothing enqueued)
     bool may_alloc = !no_alloc;
@@ -543,7 +553,6 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
                           !p.herm_load && p.store_len == p.n && !(mul && i == 0);
         tables[i] = mfma ? get_mfma_table(precision_, dir > 0, may_alloc).dev : get_twiddles(dbl, p, may_alloc).dev;
     }
-    if (no_alloc && ep.ws_elems > 0 && !ws) throw std::invalid_argument("a call that may not allocate needs a workspace");
     void *tmp = nullptr;
     if (ep.ws_elems > 0 && !ws) {
         // fft3d::forward(in, out) allocates the workspace per call (include/heffte_fft3d.h:353-360)

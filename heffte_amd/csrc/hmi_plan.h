@@ -96,7 +96,8 @@ class fft_plan {
     // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
     // overloads of fft3d (real input on forward, real output on backward).  mul (c2c only):
     // the first pass loads in[i] * mul[i] instead of in[i] (mul laid out like in).
-    // no_alloc: the call allocates nothing -- ws must be given and the device tables must be
+    // no_alloc: the call allocates nothing -- ws must be null (the workspace is the static
+    // scratch of the current device, kScratchBytes at most) and the device tables must be
     // resident (prepare); otherwise the call throws before it launches anything.  A call on a
     // stream under capture is held to the same rule for the tables (their upload is a
     // synchronous copy, which a capture cannot record).

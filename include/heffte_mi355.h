@@ -170,6 +170,21 @@ int heffte_mi355_convolve_spectrum_c2c(heffte_mi355_plan plan, int batch, void c
 int heffte_mi355_convolve_spectrum_z2z(heffte_mi355_plan plan, int batch, void const *x, void const *spectrum,
                                        void *output, void *workspace, int scale);
 
+/* No-allocation variant of heffte_mi355_forward_c2c / _z2z and heffte_mi355_backward_c2c / _z2z
+ * (direction -1 forward, +1 backward; double_precision selects complex<double>).  It has no
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The call allocates
+ * nothing: the workspace is a static device array of HMI_NOALLOC_SCRATCH_BYTES per GPU, held
+ * from the first such call on that GPU until the process ends, and the plan's device tables
+ * must have been uploaded by heffte_mi355_plan_prepare (same precision) beforehand.  A call
+ * whose workspace (batch * heffte_mi355_size_workspace(plan) complex elements) exceeds the
+ * scratch, or whose tables are not resident, fails with HMI_ERROR_INVALID_ARGUMENT /
+ * HMI_ERROR_DEVICE and enqueues nothing.  Every call on one GPU shares that GPU's scratch:
+ * calls that overlap on one GPU (other streams, other threads) race, so serialize them. */
+#define HMI_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int heffte_mi355_execute_c2c_noalloc(heffte_mi355_plan plan, int direction, int double_precision, int batch,
+                                     void const *input, void *output, int scale);
+
 /* Real-to-real transforms of an HMI_KIND_COS/SIN/COS1 plan; replace
  * rtransform::forward/backward(float/double data, workspace)
  * (include/heffte_fft3d.h:374-384,480-490 with a real scalar type). */

@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--no-error", action="store_true", help="skip the FP64 error check")
     p.add_argument("--no-batched", action="store_true", help="skip the 1000-transform mixed-size batch leg")
     p.add_argument("--no-split", action="store_true", help="skip the split-precision MFMA forms")
+    p.add_argument("--no-double", action="store_true", help="skip the 512^3 complex<double> leg")
     p.add_argument("--no-secondary", action="store_true",
                    help="skip the r2c 1024x1024x512, DCT 256^3 and 512^3 reshape legs")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -50,9 +51,36 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(n):
+def pass_times(plan, step, steps, torch, bytes_per_launch):
+    """Average launch duration of each pass kernel over `steps` calls of step(), from the
+    library's per-pass HIP events (recorded on the plan's stream)."""
+    plan.set_timing(True)
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    rep = plan.timing()
+    plan.set_timing(False)
+    out = {}
+    for name, (launches, total) in rep.items():
+        avg = total / launches
+        out[name] = {"launches": launches, "total_ms": round(total, 4), "avg_ms": round(avg, 5),
+                     "GBps": round(bytes_per_launch / (avg * 1e-3) / 1e9, 1),
+                     "frac": round(bytes_per_launch / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def seed1_input(n):
+    """n^3 complex<double> values with iid N(0, 1) real and imaginary parts (numpy seed 1)."""
+    import numpy as np
+    rng = np.random.default_rng(1)
+    return rng.standard_normal(2 * n ** 3).view(np.complex128)
+
+
+def cpu_baseline(n, x=None, y_gpu=None):
     """The FP64 oracle (a CPU restatement of heFFTe's path) on the host cores:
-    one forward + backward pair of the same transform, full size."""
+    one forward + backward pair of the same transform, full size.  When the complex<double> GPU
+    leg ran on the same input (x, seed 1), its forward output y_gpu is checked against the
+    oracle's forward output here."""
     import numpy as np
     from oracle import pyoracle
     threads = min(16, os.cpu_count() or 1)
@@ -62,18 +90,61 @@ def cpu_baseline(n):
     m = n
     b = pyoracle.Box.make((0, 0, 0), (m - 1, m - 1, m - 1), (0, 1, 2))
     bo = pyoracle.Box.make((0, 0, 0), (m - 1, m - 1, m - 1), (1, 2, 0))
-    rng = np.random.default_rng(1)
-    x = rng.standard_normal(m ** 3) + 1j * rng.standard_normal(m ** 3)
+    if x is None:
+        x = seed1_input(m)
     s = 1.0 / math.sqrt(m ** 3)
     t0 = time.perf_counter()
     y = pyoracle.fft3d_c2c(b, bo, x, -1, s)
-    del x
     z = pyoracle.fft3d_c2c(bo, b, y, +1, s)
     t1 = time.perf_counter()
+    res = {"value": round((t1 - t0) * 1e3, 1), "unit": "ms per fwd+bwd pair", "cores": threads, "kind": "port",
+           "sample": f"one full {m}^3 forward+backward pair of the FP64 oracle (complex<double>, "
+                     f"OpenMP, {threads} threads), (0,1,2)->(1,2,0), scale::symmetric"}
+    if y_gpu is not None:
+        for key, yg in y_gpu.items():
+            res[f"gpu_{key}_forward_rel_l2_vs_oracle"] = float(np.linalg.norm(yg - y) / np.linalg.norm(y))
     del y, z
-    return {"value": round((t1 - t0) * 1e3, 1), "unit": "ms per fwd+bwd pair", "cores": threads, "kind": "port",
-            "sample": f"one full {m}^3 forward+backward pair of the FP64 oracle (complex<double>, "
-                      f"OpenMP, {threads} threads), (0,1,2)->(1,2,0), scale::symmetric"}
+    return res
+
+
+def double_leg(heffte, torch, dev, plan, args, x_np):
+    """512^3 complex<double> (the second element type of the headline configuration): the same
+    boxes and protocol (in place, caller workspace, forward + backward scale::symmetric), input
+    iid N(0, 1) (numpy seed 1).  Per-pass times from the library's HIP events; the forward outputs
+    of the same input in complex<double> and complex<float> go to the CPU leg, which checks them
+    against the FP64 oracle."""
+    n = args.n
+    N = n ** 3
+    bytes_per_launch = 2.0 * N * 16
+    data = torch.from_numpy(x_np).to(dev)
+    ws = torch.empty(max(1, plan.size_workspace()), dtype=torch.complex128, device=dev)
+
+    def step():
+        plan.forward_buffered(data, data, ws, heffte.scale.symmetric)
+        plan.backward_buffered(data, data, ws, heffte.scale.symmetric)
+
+    for _ in range(args.warmup):
+        step()
+    ms = _timed(torch, step, args.steps)
+    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch)
+    dom = max(passes, key=lambda k: passes[k]["total_ms"])
+    res = {"workload": "c2c 512x512x512 complex<double>, inbox order (0,1,2) -> outbox order (1,2,0), in place, "
+                       "caller workspace, forward+backward scale::symmetric, iid N(0,1) (numpy seed 1)",
+           "ms_per_pair": round(ms, 4), "gflops_5nlog2n": round(2 * 5.0 * N * math.log2(N) / (ms * 1e-3) / 1e9, 1),
+           "dominant_kernel": dom, "dominant_avg_ms": passes[dom]["avg_ms"], "dominant_frac": passes[dom]["frac"],
+           "passes": passes, "bytes_per_launch": bytes_per_launch}
+    data.copy_(torch.from_numpy(x_np))
+    plan.forward_buffered(data, data, ws, heffte.scale.symmetric)
+    y = {"complex_double": data.cpu().numpy()}
+    del data, ws
+    torch.cuda.empty_cache()
+    d32 = torch.from_numpy(x_np).to(dev).to(torch.complex64)
+    ws32 = torch.empty(max(1, plan.size_workspace()), dtype=torch.complex64, device=dev)
+    plan.forward_buffered(d32, d32, ws32, heffte.scale.symmetric)
+    y["complex_float"] = d32.cpu().numpy().astype("complex128")
+    del d32, ws32
+    torch.cuda.empty_cache()
+    return res, y
 
 
 SMOOTH = [16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64, 80, 96, 105, 112, 128]
@@ -385,10 +456,16 @@ def main():
     ms_total = ev0.elapsed_time(ev1)
     ms_pair = ms_total / args.steps
 
-    # dominant kernel: every launch of a pair is the n-point pass (3 forward + 3 backward)
+    # every launch of a pair is an n-point pass (3 forward + 3 backward) reading and writing the
+    # whole array; the passes differ in their store layout (column chunks or contiguous lines).
+    # Each pass kernel is timed with HIP events on the plan's stream over the same number of pairs
+    # (a second timed region, so that the events do not perturb the first); the dominant kernel
+    # is the one with the largest total time.
     launches_per_pair = 6
-    avg_launch_ms = ms_pair / launches_per_pair
     bytes_per_launch = 2.0 * N * esize  # read + write of the array
+    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch)
+    dominant = max(passes, key=lambda k: passes[k]["total_ms"])
+    avg_launch_ms = passes[dominant]["avg_ms"]
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     gflops = 2 * 5.0 * N * math.log2(N) / (ms_pair * 1e-3) / 1e9
 
@@ -414,8 +491,12 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": MEASURED_TRAFFIC.get((n, args.dtype), (None,))[0],
                      "traffic_source": MEASURED_TRAFFIC.get((n, args.dtype), (None, None))[1],
-                     "kernel": f"fft_uniform {n}-point pass (all 6 launches of a pair)",
-                     "avg_launch_ms": round(avg_launch_ms, 4), "bytes_per_launch": bytes_per_launch},
+                     "kernel": dominant, "avg_launch_ms": round(avg_launch_ms, 5),
+                     "bytes_per_launch": bytes_per_launch,
+                     "timing": "HIP events around each pass launch on the plan's stream, "
+                               f"{args.steps} pairs after the timed region",
+                     "passes": passes,
+                     "pair_from_passes_ms": round(sum(v["total_ms"] for v in passes.values()) / args.steps, 4)},
     }
 
     # split-precision MFMA forms of the same transform (HMI_PRECISION_SPLIT_FP16 / _BF16)
@@ -433,8 +514,12 @@ def main():
             e1.record(stream)
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / args.steps
+            fp = pass_times(plan, step, args.steps, torch, bytes_per_launch)
+            dom = max(fp, key=lambda k: fp[k]["total_ms"])
             entry = {"ms_per_pair": round(ms, 4), "speedup_vs_fp32": round(ms_pair / ms, 3),
-                     "hbm_frac": round(bytes_per_launch / (ms / launches_per_pair * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                     "hbm_frac_pair_over_6": round(bytes_per_launch / (ms / launches_per_pair * 1e-3) / 1e9 /
+                                                   HBM_PEAK_GBS, 4),
+                     "dominant_kernel": dom, "dominant_frac": fp[dom]["frac"], "passes": fp}
             if x0 is not None:
                 yf = torch.empty_like(x0)
                 plan.forward_buffered(x0, yf, ws, heffte.scale.none)
@@ -512,10 +597,18 @@ def main():
         out["convolve_512_c64"] = convolve_leg(heffte, torch, dev, max(3, args.steps // 2))
         data = ws = None
 
+    x_np = y_gpu = None
+    if not args.no_double and n == 512 and args.dtype == "complex64":
+        del data, ws
+        torch.cuda.empty_cache()
+        x_np = seed1_input(n)
+        out["c2c_512_complex_double"], y_gpu = double_leg(heffte, torch, dev, plan, args, x_np)
+        data = ws = None
+
     if not args.no_cpu:
         del data, ws
         torch.cuda.empty_cache()
-        out["cpu_baseline"] = cpu_baseline(n)
+        out["cpu_baseline"] = cpu_baseline(n, x_np, y_gpu)
 
     print(json.dumps(out), flush=True)
 

@@ -63,6 +63,9 @@ def _load():
         fn = getattr(lib, "heffte_mi355_" + f)
         fn.argtypes = [plan_p, ctypes.c_int, vp, vp, vp, ctypes.c_int]
         fn.restype = ctypes.c_int
+    lib.heffte_mi355_plan_set_timing.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_plan_timing_report.argtypes = [plan_p]
+    lib.heffte_mi355_plan_timing_report.restype = ctypes.c_char_p
     lib.heffte_mi355_execute_c2c_noalloc.argtypes = [plan_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp, vp,
                                                      ctypes.c_int]
     lib.heffte_mi355_execute_c2c_noalloc.restype = ctypes.c_int
@@ -107,7 +110,8 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_batch_backward_z2z", "heffte_mi355_set_precision", "heffte_mi355_batch_set_autotune",
     "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
     "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z", "heffte_mi355_plan_prepare",
-    "heffte_mi355_batch_graph_captures", "heffte_mi355_execute_c2c_noalloc",
+    "heffte_mi355_batch_graph_captures", "heffte_mi355_execute_c2c_noalloc", "heffte_mi355_plan_set_timing",
+    "heffte_mi355_plan_timing_report",
 )
 
 
@@ -227,6 +231,23 @@ class fft_plan:
         """Uploads the device tables of every transform variant in one precision now, so that
         later calls (e.g. under stream capture) upload nothing."""
         _check(lib.heffte_mi355_plan_prepare(self._h, 1 if double_precision else 0), "prepare")
+
+    def set_timing(self, enable=True):
+        """Starts (or stops) recording a hipEvent around every pass launch of this plan's calls;
+        either drops the records so far."""
+        _check(lib.heffte_mi355_plan_set_timing(self._h, 1 if enable else 0), "set_timing")
+
+    def timing(self):
+        """Per pass kernel since set_timing(True): {name: (launches, total ms)}; waits for the
+        recorded events."""
+        txt = lib.heffte_mi355_plan_timing_report(self._h)
+        if txt is None:
+            raise heffte_input_error(f"timing report failed: {last_error()}")
+        out = {}
+        for line in txt.decode().splitlines():
+            name, launches, ms = line.split("\t")
+            out[name] = (int(launches), float(ms))
+        return out
 
     def set_stream(self, stream):
         """stream: a torch.cuda.Stream, a raw hipStream_t address, or None (default stream)."""

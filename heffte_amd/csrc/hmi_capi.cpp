@@ -186,6 +186,22 @@ int heffte_mi355_plan_prepare(heffte_mi355_plan plan, int double_precision) {
     return guarded([&] { plan->impl->prepare(double_precision != 0); });
 }
 
+int heffte_mi355_plan_set_timing(heffte_mi355_plan plan, int enable) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    return guarded([&] { plan->impl->set_timing(enable != 0); });
+}
+
+char const *heffte_mi355_plan_timing_report(heffte_mi355_plan plan) {
+    static thread_local std::string report;
+    report.clear();
+    if (!valid(plan)) {
+        fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+        return nullptr;
+    }
+    const int rc = guarded([&] { report = plan->impl->timing_report(); });
+    return rc == HMI_SUCCESS ? report.c_str() : nullptr;
+}
+
 long long heffte_mi355_size_convolve_workspace(heffte_mi355_plan plan) {
     return valid(plan) && plan->impl->get_kind() == hmi::kind::c2c ? plan->impl->convolve_workspace() : -1;
 }

@@ -121,6 +121,7 @@ void fft_plan::prepare(bool dbl) {
 }
 
 fft_plan::~fft_plan() {
+    clear_timing();
     for (auto &kv : tw_)
         if (kv.second.dev) (void)hipFree(kv.second.dev);
     for (auto &tp : mfma_tw_)
@@ -479,6 +480,26 @@ const char *kNotResident =
     "point) needs heffte_mi355_plan_prepare on this plan and precision first";
 }
 
+namespace {
+// Name of the kernel a pass launches, in the form the profilers print the template.
+std::string pass_kernel_name(const pass &p, bool dbl, bool mfma, int precision, const pass_args &a) {
+    const char *R = dbl ? "double" : "float";
+    std::string s;
+    if (mfma) {
+        s = std::string("fft512_mfma<") + (precision == kSplitF16 ? "kSplitF16" : "kSplitBF16") + ", " +
+            (a.load_colfast ? "true" : "false") + ", " + (a.store_colfast ? "true" : "false") + ">";
+    } else if (p.fast) {
+        s = std::string("fft_uniform<") + R + ", radices<";
+        for (int i = 0; i < p.fast->nstages; i++) s += (i ? ", " : "") + std::to_string(p.fast->radix[i]);
+        s += ">, " + std::to_string(p.fast->E) + ", " + std::to_string(p.fast->T) + ", " +
+             (a.stream ? "true" : "false") + ">";
+    } else {
+        s = std::string("fft_generic<") + R + "> n=" + std::to_string(p.n);
+    }
+    return s;
+}
+}  // namespace
+
 const twiddles &fft_plan::get_twiddles(bool dbl, const pass &p, bool may_alloc) {
     const int n = p.n;
     const long long key = ((long long)n << 8) | (dbl ? 1 : 0) | (p.fast ? 2 : 0) | (p.identity ? 4 : 0) |
@@ -572,6 +593,15 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         return e + (long long)(batch - 1) * (L.n[0] * L.n[1] * L.n[2]) + 1;
     };
     const bool debug = std::getenv("HMI_DEBUG_PLAN") != nullptr;
+    // per-pass timing: one event before the first pass and one after each pass
+    std::unique_lock<std::mutex> tlock(mtx_, std::defer_lock);
+    hipEvent_t tprev = nullptr;
+    if (timing_) {
+        tlock.lock();
+        check_hip(hipEventCreate(&tprev), "hipEventCreate");
+        timing_events_.push_back(tprev);
+        check_hip(hipEventRecord(tprev, stream_), "hipEventRecord");
+    }
     for (const pass &p : ep.passes) {
         // every element a pass touches lies inside the buffer it names
         if (extent(p.lin) > capacity(p.src) || extent(p.lout) > capacity(p.dst))
@@ -640,9 +670,54 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
             const size_t lds = 2 * (size_t)p.gen.lines * p.n * (dbl ? 16 : 8);
             check_hip(launch_generic(dbl, a, p.gen, nblocks, lds, stream_), "generic FFT pass");
         }
+        if (tprev) {
+            const std::string name = pass_kernel_name(p, dbl, mfma, precision_, a);
+            int id = (int)(std::find(timing_names_.begin(), timing_names_.end(), name) - timing_names_.begin());
+            if (id == (int)timing_names_.size()) timing_names_.push_back(name);
+            hipEvent_t e = nullptr;
+            check_hip(hipEventCreate(&e), "hipEventCreate");
+            timing_events_.push_back(e);
+            check_hip(hipEventRecord(e, stream_), "hipEventRecord");
+            timing_launches_.push_back({id, {tprev, e}});
+            tprev = e;
+        }
     }
     if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
 }
+
+void fft_plan::clear_timing() {
+    for (hipEvent_t e : timing_events_) (void)hipEventDestroy(e);
+    timing_events_.clear();
+    timing_launches_.clear();
+}
+
+void fft_plan::set_timing(bool on) {
+    std::lock_guard<std::mutex> lock(mtx_);
+    clear_timing();
+    timing_ = on;
+}
+
+std::string fft_plan::timing_report() {
+    std::lock_guard<std::mutex> lock(mtx_);
+    std::vector<long long> count(timing_names_.size(), 0);
+    std::vector<double> total(timing_names_.size(), 0.0);
+    for (const auto &rec : timing_launches_) {
+        check_hip(hipEventSynchronize(rec.second.second), "hipEventSynchronize");
+        float ms = 0.0f;
+        check_hip(hipEventElapsedTime(&ms, rec.second.first, rec.second.second), "hipEventElapsedTime");
+        count[rec.first] += 1;
+        total[rec.first] += ms;
+    }
+    std::string out;
+    char line[64];
+    for (size_t i = 0; i < timing_names_.size(); i++) {
+        if (!count[i]) continue;
+        std::snprintf(line, sizeof(line), "\t%lld\t%.6f\n", count[i], total[i]);
+        out += timing_names_[i] + line;
+    }
+    return out;
+}
+
 
 void fft_plan::convolve(bool dbl, int batch, const void *x, const void *y, void *out, void *ws, int scaling) {
     if (kind_ != kind::c2c) throw std::invalid_argument("convolution needs a c2c plan");

@@ -120,6 +120,14 @@ class fft_plan {
     // Plan (without running it): exposed for tests of the planner.
     const exec_plan &get_exec_plan(int dir, bool dbl, bool in_real, bool out_real, bool in_place);
 
+    // Per-pass timing: while on, every call records a hipEvent on the plan's stream before its
+    // first pass and after each pass, so the time between two consecutive events is the
+    // duration of one pass launch.  timing_report() waits for the recorded events and returns
+    // one line per pass kernel, "name<TAB>launches<TAB>total ms", aggregated over every call
+    // since timing was switched on; switching it off (or on again) drops the records.
+    void set_timing(bool on);
+    std::string timing_report();
+
   private:
     exec_plan build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const;
     void configure(pass &p, bool dbl) const;
@@ -137,6 +145,12 @@ class fft_plan {
     twiddles mfma_tw_[2][2] = {};  // [split FP16, split BF16][forward, backward]
     std::map<int, exec_plan> plans_;
     std::map<long long, twiddles> tw_;
+    // per-pass timing records: launch i ran between events marks_[i].first and .second
+    bool timing_ = false;
+    std::vector<std::string> timing_names_;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timing_launches_;
+    std::vector<hipEvent_t> timing_events_;
+    void clear_timing();
 };
 
 // Runtime radix plan of length n for the generic tile (stage/root/post offsets

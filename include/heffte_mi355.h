@@ -121,6 +121,16 @@ int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision);
  * again after heffte_mi355_set_precision selects a split-precision form. */
 int heffte_mi355_plan_prepare(heffte_mi355_plan plan, int double_precision);
 
+/* Per-pass timing (diagnostic, no heFFTe counterpart; heFFTe's add_trace/finalize_tracing,
+ * include/heffte_trace.h, record host-side events per stage instead).  enable = 1 makes every
+ * later call on the plan record a hipEvent on the plan's stream before its first pass and after
+ * each pass; 0 stops; either drops the records so far.  The report waits for the recorded events
+ * and returns one line per pass kernel, "kernel name<TAB>launches<TAB>total milliseconds", over
+ * every call since timing was enabled (NULL on error; the text stays valid until the next report
+ * on the calling thread). */
+int heffte_mi355_plan_set_timing(heffte_mi355_plan plan, int enable);
+char const *heffte_mi355_plan_timing_report(heffte_mi355_plan plan);
+
 /* Forward transforms; replace heffte_forward_{s2c,c2c,d2z,z2z}_buffered
  * (include/heffte_c.h:164-179) with a batch size.  workspace may be NULL, in
  * which case it is allocated for the call (fft3d::forward(in, out),

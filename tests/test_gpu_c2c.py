@@ -184,3 +184,29 @@ def test_stream_binding(hmi, cuda, oracle):
     s.synchronize()
     ref = oracle_c2c(oracle, size, (0, 1, 2), (0, 1, 2), x, -1, 1.0)
     assert fc.rel_l2(ty.cpu().numpy(), ref) <= 1e-13
+
+
+def test_pass_timing_report(hmi, cuda, oracle):
+    """Per-pass HIP-event timing: one record per pass launch, named by its kernel; results are
+    unchanged while timing is on, and switching it off drops the records."""
+    import torch
+    size = (64, 48, 40)
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], [0, 1, 2])
+    outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], [1, 2, 0])
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    x = fc.uniform_complex(int(np.prod(size)), 6)
+    tx = fc.to_gpu(x, np.complex64)
+    ty = torch.empty_like(tx)
+    ws = torch.empty(plan.size_workspace(), dtype=tx.dtype, device=cuda)
+    plan.set_timing(True)
+    for _ in range(3):
+        plan.forward_buffered(tx, ty, ws)
+    rep = plan.timing()
+    assert sum(v[0] for v in rep.values()) == 9, rep  # 3 calls x 3 passes
+    assert all(v[1] > 0 for v in rep.values())
+    assert all(k.startswith(("fft_uniform<float", "fft_generic<float")) for k in rep)
+    ref = oracle_c2c(oracle, size, (0, 1, 2), (1, 2, 0), x, -1, 1.0)
+    assert fc.rel_l2(ty.cpu().numpy(), ref) <= 5e-6
+    plan.set_timing(False)
+    plan.forward_buffered(tx, ty, ws)
+    assert plan.timing() == {}

@@ -36,9 +36,19 @@ def common_flags():
     ]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [hipcc()] + common_flags() + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+# AddressSanitizer and UndefinedBehaviorSanitizer for the host code only (planner, C ABI, batch
+# host logic): every -fsanitize= goes to the host compilation (-Xarch_host); device code is
+# built as usual.
+SAN_HOST_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+                  "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+SAN_LIB = os.path.join(HERE, "libheffte_mi355_san.so")
+
+
+def _compile(src, sanitize=False):
+    bdir = os.path.join(BUILD, "san") if sanitize else BUILD
+    obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc()] + common_flags() + (SAN_HOST_FLAGS if sanitize else []) + \
+        ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile of {src} failed:\n{r.stderr}")
@@ -47,22 +57,27 @@ def _compile(src):
     return obj, warn
 
 
-def build(verbose=False):
-    os.makedirs(BUILD, exist_ok=True)
+def build(verbose=False, sanitize=False):
+    """Compiles and links the library; sanitize=True builds libheffte_mi355_san.so with the host
+    code under ASan/UBSan instead (CPU tests only: load it with the sanitizer runtime preloaded)."""
+    os.makedirs(os.path.join(BUILD, "san") if sanitize else BUILD, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        results = list(ex.map(_compile, SOURCES))
+        results = list(ex.map(lambda src: _compile(src, sanitize), SOURCES))
     objs = [o for o, _ in results]
     if verbose:
         for _, w in results:
             if w:
                 print(w, file=sys.stderr)
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    target = SAN_LIB if sanitize else LIB
+    tmp = target + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + \
+        (["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined", "-shared-libasan"]
+         if sanitize else []) + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, target)
+    return target
 
 
 FRONTEND_TEST_SRC = os.path.join(HERE, "..", "tests", "cpp", "test_frontend.cpp")

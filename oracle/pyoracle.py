@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# HEFFTE_ORACLE_LIBRARY selects another build of the oracle (the sanitized one of heffte_amd/sanitize.py)
+_LIB_PATH = os.environ.get("HEFFTE_ORACLE_LIBRARY") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 

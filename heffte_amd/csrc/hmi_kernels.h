@@ -10,15 +10,17 @@ namespace hmi {
 
 // Largest dynamic LDS the generic pass may request (gfx950: 160 KiB per workgroup).
 constexpr int kGenericMaxLds = 160 * 1024;
-// pass outputs above this size use non-temporal loads and stores (larger than the MALL)
+// pass outputs above this size use non-temporal stores (larger than the MALL), and their loads
+// too when a side of the pass walks the column axis (pass_args::stream)
 constexpr long long kStreamBytes = 256LL << 20;
+enum : int { kStreamStores = 1, kStreamLoadsStores = 3 };
 
 using fast_launcher = hipError_t (*)(const pass_args &, long long nblocks, hipStream_t);
 
 // A compiled uniform pass: length n, E elements per thread, T lines per block.
 struct fast_kernel_info {
     bool dbl;
-    int n, E, T;
+    int n, E, T, PS;
     int radix[8];
     int nstages;
     fast_launcher launch;

@@ -45,6 +45,31 @@ template <int... Rs> struct radices {
 
 template <int PS> __device__ __forceinline__ int padj(int j) { return j + (j >> PS); }
 
+// LDS tile of the uniform pass: element j of line l.  Rows of n elements padded by one element
+// per 2^ps (+1), or, for the 512-point tiles with 8 elements per thread, rows of n + 2 elements
+// with the low bits of j XOR-swizzled by its higher bits.  LDS serves 8-byte accesses 32 lanes
+// per cycle over its 64 four-byte banks (16-byte accesses 16 lanes): the swizzle gives every
+// exchange of those tiles distinct banks -- the stage writes (one line per wave: positions
+// 8t + q and 64a + b + 8q), the reads of positions t + 64e in either thread mapping, and the
+// column-walking mapping, where the 16 lines of a half-wave differ by 2 rows of 32 slots.
+template <int N, int E, int PS, int BYTES> struct lds_map {
+    static constexpr bool swz = (N == 512 && E == 8);
+    static constexpr int RS = swz ? N + 2 : N + (N >> PS) + 1;
+    __device__ static __forceinline__ int idx(int l, int j) {
+        if constexpr (swz) {
+            if constexpr (BYTES == 8) {
+                const int h = j >> 5;
+                return l * RS + (j ^ ((h & 7) | (((h >> 1) & 3) << 3)));
+            } else {
+                const int h = j >> 4;
+                return l * RS + (j ^ ((h & 7) | (((h >> 2) & 1) << 3)));
+            }
+        } else {
+            return l * RS + padj<PS>(j);
+        }
+    }
+};
+
 // Row stride (complex elements of `bytes` bytes) of the LDS tile of an n-point line with one pad
 // element per 2^ps, for tiles of t lines.
 constexpr int tile_row_stride(int n, int ps, int t, int bytes) { return n + (n >> ps) + 1 + 0 * t * bytes; }
@@ -52,6 +77,7 @@ constexpr int tile_row_stride(int n, int ps, int t, int bytes) { return n + (n >
 template <int S, typename RL, int E, int P, int RS, int PS, typename C>
 __device__ __forceinline__ void run_stages(C (&v)[E], C *lds, const C *__restrict__ tw, int l, int t, int l2,
                                            int t2) {
+    using LM = lds_map<RL::n(), E, PS, sizeof(C)>;
     constexpr int R = RL::at(S);
     constexpr int Ls = RL::ls(S);
     constexpr int NB = E / R;
@@ -77,13 +103,13 @@ __device__ __forceinline__ void run_stages(C (&v)[E], C *lds, const C *__restric
         } else {
             const int base = (b / Ls) * Ls * R + k;
 #pragma unroll
-            for (int q = 0; q < R; q++) lds[l * RS + padj<PS>(base + q * Ls)] = x[q];
+            for (int q = 0; q < R; q++) lds[LM::idx(l, base + q * Ls)] = x[q];
         }
     }
     if constexpr (!last) {
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < E; e++) v[e] = lds[l2 * RS + padj<PS>(t2 + P * e)];
+        for (int e = 0; e < E; e++) v[e] = lds[LM::idx(l2, t2 + P * e)];
         __syncthreads();
         run_stages<S + 1, RL, E, P, RS, PS>(v, lds, tw, l2, t2, l2, t2);
     }
@@ -100,12 +126,14 @@ __device__ __forceinline__ long long line_offset(unsigned L, const pass_args &a,
 }
 
 // One tile (T lines) of an n-point pass; the block has exactly (n / E) * T threads.
-template <typename R, typename RL, int E, int T, int PS, bool NT = false>
+template <typename R, typename RL, int E, int T, int PS, int NTM = 0>
 __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *lds) {
     using C = typename cvec<R>::type;
     constexpr int N = RL::n();
     constexpr int P = N / E;
-    constexpr int RS = tile_row_stride(N, PS, T, sizeof(C));
+    using LM = lds_map<N, E, PS, sizeof(C)>;
+    constexpr int RS = LM::RS;
+    constexpr bool NTL = (NTM & 2) != 0, NTS = (NTM & 1) != 0;
 
     const int tid = threadIdx.x;
     int l, t, l2, t2;
@@ -133,12 +161,12 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             if (a.in_sa == 1 && (reinterpret_cast<size_t>(a.in) & (sizeof(C) - 1)) == 0) {
 #pragma unroll
                 for (int e = 0; e < E; e++)
-                    v[e] = valid ? ldg<NT>(reinterpret_cast<const C *>(src + base) + (t + P * e)) : cmk<C>(R(0), R(0));
+                    v[e] = valid ? ldg<NTL>(reinterpret_cast<const C *>(src + base) + (t + P * e)) : cmk<C>(R(0), R(0));
             } else {
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const long long m2 = 2LL * (t + P * e);
-                    v[e] = valid ? cmk<C>(ldg<NT>(&src[base + m2 * a.in_sa]), ldg<NT>(&src[base + (m2 + 1) * a.in_sa]))
+                    v[e] = valid ? cmk<C>(ldg<NTL>(&src[base + m2 * a.in_sa]), ldg<NTL>(&src[base + (m2 + 1) * a.in_sa]))
                                  : cmk<C>(R(0), R(0));
                 }
             }
@@ -149,15 +177,15 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const C *__restrict__ src = static_cast<const C *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++)
-                lds[l * RS + padj<PS>(t + P * e)] =
-                    valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
-            if (t == 0) lds[l * RS + padj<PS>(N)] = valid ? ldg<NT>(&src[base + (long long)N * a.in_sa]) : cmk<C>(R(0), R(0));
+                lds[LM::idx(l, t + P * e)] =
+                    valid ? ldg<NTL>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
+            if (t == 0) lds[LM::idx(l, N)] = valid ? ldg<NTL>(&src[base + (long long)N * a.in_sa]) : cmk<C>(R(0), R(0));
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int k = t + P * e;
-                C xk = lds[l * RS + padj<PS>(k)];
-                C xm = lds[l * RS + padj<PS>(N - k)];
+                C xk = lds[LM::idx(l, k)];
+                C xm = lds[LM::idx(l, N - k)];
                 if (k == 0) { xk.y = R(0); xm.y = R(0); }
                 const C w = tw_half[k];  // exp(-i pi k / N); w^{-k} = conj(w)
                 const C sum = cmk<C>(xk.x + xm.x, xk.y - xm.y);
@@ -174,7 +202,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             for (int e = 0; e < E; e++) {
                 const int n = t + P * e;
                 const int j = (n < (N + 1) / 2) ? 2 * n : 2 * (N - 1 - n) + 1;
-                R x = valid ? ldg<NT>(&src[base + (long long)j * a.in_sa]) : R(0);
+                R x = valid ? ldg<NTL>(&src[base + (long long)j * a.in_sa]) : R(0);
                 if (sine && (j & 1)) x = -x;
                 v[e] = cmk<C>(x, R(0));
             }
@@ -186,8 +214,8 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
                 const int k = t + P * e;
                 const int jk = sine ? N - 1 - k : k;
                 const int jm = sine ? k - 1 : N - k;
-                const R xk = valid ? ldg<NT>(&src[base + (long long)jk * a.in_sa]) : R(0);
-                const R xm = (valid && k > 0) ? ldg<NT>(&src[base + (long long)jm * a.in_sa]) : R(0);
+                const R xk = valid ? ldg<NTL>(&src[base + (long long)jk * a.in_sa]) : R(0);
+                const R xm = (valid && k > 0) ? ldg<NTL>(&src[base + (long long)jm * a.in_sa]) : R(0);
                 const C w = tw_half[k];  // exp(-i pi k / 2N)
                 // W = conj(w) (xk - i xm); the inverse DFT runs as conj(DFT(conj W))
                 const C W = cmk<C>(w.x * xk - w.y * xm, -w.x * xm - w.y * xk);
@@ -200,20 +228,20 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             for (int e = 0; e < E; e++) {
                 const int j = t + P * e;
                 const int jj = (j <= N / 2) ? j : N - j;
-                v[e] = cmk<C>(valid ? ldg<NT>(&src[base + (long long)jj * a.in_sa]) : R(0), R(0));
+                v[e] = cmk<C>(valid ? ldg<NTL>(&src[base + (long long)jj * a.in_sa]) : R(0), R(0));
             }
         } else if (a.in_real) {
             const R *__restrict__ src = static_cast<const R *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++)
-                v[e] = valid ? cmk<C>(ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]), R(0)) : cmk<C>(R(0), R(0));
+                v[e] = valid ? cmk<C>(ldg<NTL>(&src[base + (long long)(t + P * e) * a.in_sa]), R(0)) : cmk<C>(R(0), R(0));
         } else if (a.herm_load) {
             const C *__restrict__ src = static_cast<const C *>(a.in);
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int pos = t + P * e;
                 const bool mirror = pos > N / 2;
-                C x = valid ? ldg<NT>(&src[base + (long long)(mirror ? N - pos : pos) * a.in_sa]) : cmk<C>(R(0), R(0));
+                C x = valid ? ldg<NTL>(&src[base + (long long)(mirror ? N - pos : pos) * a.in_sa]) : cmk<C>(R(0), R(0));
                 v[e] = cmk<C>(x.x, (mirror ? -sg : sg) * x.y);
             }
         } else {
@@ -223,7 +251,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const long long off = base + (long long)(t + P * e) * a.in_sa;
-                    C y = valid ? ldg<NT>(&src[off]) : cmk<C>(R(0), R(0));
+                    C y = valid ? ldg<NTL>(&src[off]) : cmk<C>(R(0), R(0));
                     C x = cmul(y, y);
                     v[e] = cmk<C>(x.x, sg * x.y);
                 }
@@ -233,13 +261,13 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const long long off = base + (long long)(t + P * e) * a.in_sa;
-                    C x = valid ? cmul(ldg<NT>(&src[off]), ldg<NT>(&mul[off])) : cmk<C>(R(0), R(0));
+                    C x = valid ? cmul(ldg<NTL>(&src[off]), ldg<NTL>(&mul[off])) : cmk<C>(R(0), R(0));
                     v[e] = cmk<C>(x.x, sg * x.y);
                 }
             } else {
 #pragma unroll
                 for (int e = 0; e < E; e++) {
-                    C x = valid ? ldg<NT>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
+                    C x = valid ? ldg<NTL>(&src[base + (long long)(t + P * e) * a.in_sa]) : cmk<C>(R(0), R(0));
                     v[e] = cmk<C>(x.x, sg * x.y);
                 }
             }
@@ -250,10 +278,10 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     if constexpr (RL::count == 1) {
         if (a.load_colfast != a.store_colfast) {
 #pragma unroll
-            for (int e = 0; e < E; e++) lds[l * RS + padj<PS>(t + P * e)] = v[e];
+            for (int e = 0; e < E; e++) lds[LM::idx(l, t + P * e)] = v[e];
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < E; e++) v[e] = lds[l2 * RS + padj<PS>(t2 + P * e)];
+            for (int e = 0; e < E; e++) v[e] = lds[LM::idx(l2, t2 + P * e)];
             __syncthreads();
         }
     }
@@ -265,20 +293,20 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     if (a.mode == kModeR2C) {
         // Z = DFT_N(z); X[k] = E[k] + w^k O[k], E = (Z[k] + conj Z[N-k]) / 2, O = -i (Z[k] - conj Z[N-k]) / 2
 #pragma unroll
-        for (int e = 0; e < E; e++) lds[l2 * RS + padj<PS>(t2 + P * e)] = v[e];
+        for (int e = 0; e < E; e++) lds[LM::idx(l2, t2 + P * e)] = v[e];
         __syncthreads();
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int k = t2 + P * e;
             const C zk = v[e];
-            const C zm = lds[l2 * RS + padj<PS>(k == 0 ? 0 : N - k)];
+            const C zm = lds[LM::idx(l2, k == 0 ? 0 : N - k)];
             const C ev = cmk<C>(R(0.5) * (zk.x + zm.x), R(0.5) * (zk.y - zm.y));
             const C od = cmk<C>(R(0.5) * (zk.y + zm.y), R(-0.5) * (zk.x - zm.x));
             const C w = tw_half[k];
             const C x = cadd(ev, cmul(w, od));
-            if (svalid) stg<NT>(&dst[obase + (long long)k * a.out_sa], cmk<C>(x.x * s, x.y * s));
-            if (k == 0 && svalid) stg<NT>(&dst[obase + (long long)N * a.out_sa], cmk<C>((ev.x - od.x) * s, (ev.y - od.y) * s));
+            if (svalid) stg<NTS>(&dst[obase + (long long)k * a.out_sa], cmk<C>(x.x * s, x.y * s));
+            if (k == 0 && svalid) stg<NTS>(&dst[obase + (long long)N * a.out_sa], cmk<C>((ev.x - od.x) * s, (ev.y - od.y) * s));
         }
     } else if (!svalid) {
         // column beyond the box: nothing to store
@@ -290,7 +318,7 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const int k = t2 + P * e;
             const C w = tw_half[k];
             const R y = R(2) * (w.x * v[e].x - w.y * v[e].y);
-            stg<NT>(&dst[obase + (long long)(sine ? N - 1 - k : k) * a.out_sa], y * s);
+            stg<NTS>(&dst[obase + (long long)(sine ? N - 1 - k : k) * a.out_sa], y * s);
         }
     } else if (a.mode == kModeDCT3 || a.mode == kModeDST3) {
         const bool sine = (a.mode == kModeDST3);
@@ -301,52 +329,53 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
             const int j = (m < (N + 1) / 2) ? 2 * m : 2 * (N - 1 - m) + 1;
             R y = R(2) * v[e].x;
             if (sine && (j & 1)) y = -y;
-            stg<NT>(&dst[obase + (long long)j * a.out_sa], y * s);
+            stg<NTS>(&dst[obase + (long long)j * a.out_sa], y * s);
         }
     } else if (a.mode == kModeDCT1 || a.mode == kModeDCT1B) {
         R *__restrict__ dst = static_cast<R *>(a.out);
         const R f = (a.mode == kModeDCT1B) ? R(2) * s : s;
 #pragma unroll
         for (int e = 0; e < E; e++)
-            if (t2 + P * e < a.store_len) stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * f);
+            if (t2 + P * e < a.store_len) stg<NTS>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * f);
     } else if (a.mode == kModeC2R) {
         R *__restrict__ dst = static_cast<R *>(a.out);
         if (a.out_sa == 1 && (reinterpret_cast<size_t>(a.out) & (sizeof(C) - 1)) == 0) {
 #pragma unroll
             for (int e = 0; e < E; e++)
-                stg<NT>(reinterpret_cast<C *>(dst + obase) + (t2 + P * e), cmk<C>(v[e].x * s, -v[e].y * s));
+                stg<NTS>(reinterpret_cast<C *>(dst + obase) + (t2 + P * e), cmk<C>(v[e].x * s, -v[e].y * s));
         } else {
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const long long m2 = 2LL * (t2 + P * e);
-                stg<NT>(&dst[obase + m2 * a.out_sa], v[e].x * s);
-                stg<NT>(&dst[obase + (m2 + 1) * a.out_sa], -v[e].y * s);
+                stg<NTS>(&dst[obase + m2 * a.out_sa], v[e].x * s);
+                stg<NTS>(&dst[obase + (m2 + 1) * a.out_sa], -v[e].y * s);
             }
         }
     } else if (a.out_real) {
         R *__restrict__ dst = static_cast<R *>(a.out);
 #pragma unroll
-        for (int e = 0; e < E; e++) stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * s);
+        for (int e = 0; e < E; e++) stg<NTS>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], v[e].x * s);
     } else if (a.store_len < N) {
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
         for (int e = 0; e < E; e++)
             if (t2 + P * e < a.store_len)
-                stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
+                stg<NTS>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
     } else {
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
         for (int e = 0; e < E; e++)
-            stg<NT>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
+            stg<NTS>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
     }
 }
 
-template <typename R, typename RL, int E, int T, int PS, bool NT>
+// NTM: bit 0 = non-temporal stores, bit 1 = non-temporal loads of the pass data
+template <typename R, typename RL, int E, int T, int PS, int NTM>
 __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     using C = typename cvec<R>::type;
-    constexpr int RS = tile_row_stride(RL::n(), PS, T, sizeof(C));
+    constexpr int RS = lds_map<RL::n(), E, PS, sizeof(C)>::RS;
     __shared__ C lds[T * RS];
-    uniform_tile<R, RL, E, T, PS, NT>(a, blockIdx.x, lds);
+    uniform_tile<R, RL, E, T, PS, NTM>(a, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------- generic pass
@@ -776,11 +805,14 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
 
 #define HMI_FAST(TYPE, N, E, T, PS, ...)                                                                  \
     static hipError_t launch_##TYPE##_##N##_##T(const pass_args &a, long long nblocks, hipStream_t s) {    \
-        if (a.stream)                                                                                      \
-            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, true>                                        \
+        if (a.stream == kStreamStores)                                                                     \
+            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, kStreamStores>                               \
+                <<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a);                                 \
+        else if (a.stream)                                                                                 \
+            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, kStreamLoadsStores>                          \
                 <<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a);                                 \
         else                                                                                               \
-            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, false>                                       \
+            fft_uniform<TYPE, radices<__VA_ARGS__>, E, T, PS, 0>                                           \
                 <<<dim3((unsigned)nblocks), dim3((N / E) * T), 0, s>>>(a);                                 \
         return hipGetLastError();                                                                          \
     }
@@ -836,7 +868,7 @@ __global__ void __launch_bounds__(256) permute_kernel(const E *__restrict__ src,
 HMI_FAST_TABLE(HMI_FAST)
 
 #define HMI_FAST_ENTRY(TYPE, N, E, T, PS, ...) \
-    {sizeof(TYPE) == 8, N, E, T, {__VA_ARGS__}, radices<__VA_ARGS__>::count, &launch_##TYPE##_##N##_##T},
+    {sizeof(TYPE) == 8, N, E, T, PS, {__VA_ARGS__}, radices<__VA_ARGS__>::count, &launch_##TYPE##_##N##_##T},
 
 static const fast_kernel_info kFast[] = {HMI_FAST_TABLE(HMI_FAST_ENTRY)};
 

@@ -54,7 +54,8 @@ struct pass_args {
     int store_len; // positions >= store_len along the FFT axis are not stored (r2c half spectrum)
     int herm_load; // 1: positions > n/2 are read as conj(x[n - pos]) (c2r Hermitian extension)
     int mode;      // kModeC2C, kModeR2C or kModeC2R (uniform kernel only)
-    int stream;    // 1: non-temporal loads and stores of the data (outputs larger than the MALL)
+    int stream;    // non-temporal data accesses (outputs larger than the MALL): 0 none,
+                   // kStreamStores (1) stores only, kStreamLoadsStores (3) loads and stores
     const void *mul;  // non-null: every loaded element is first multiplied by the element of this
                       // array at the same offset (same layout and type as in; plain c2c loads only)
     double scale;  // multiplies every stored value (cast to the pass precision)

@@ -113,6 +113,138 @@ def test_512_float_linearity_and_plane_wave(hmi, cuda, field512):
     assert torch.linalg.vector_norm(f).item() <= 1e-5 * N ** 3
 
 
+def plane_wave(torch, cuda, k, dtype, sign=1):
+    """exp(sign 2 pi i (k . r) / N) on the N^3 grid in order (0,1,2) (axis 0 fastest)."""
+    i = torch.arange(N, device=cuda, dtype=torch.float64)
+    ph = (k[0] * i[None, None, :] + k[1] * i[None, :, None] + k[2] * i[:, None, None]) / N
+    return torch.exp(sign * 2j * math.pi * ph).reshape(-1).to(dtype)
+
+
+def test_512_double_plane_waves_forward_and_backward(hmi, cuda):
+    """complex<double> at full size: exp(+2 pi i k.r / N) forwards to N^3 in the one bin k of the
+    outbox order (1,2,0) (a conjugated spectrum would put it at -k, a wrong output permutation at
+    another index), and a single bin backwards to exp(+2 pi i k.r / N) in the inbox order."""
+    import torch
+    plan = c2c_plan(hmi)
+    k = (5, 211, 37)
+    pw = plane_wave(torch, cuda, k, torch.complex128)
+    f = torch.empty_like(pw)
+    plan.forward(pw, f)
+    peak = k[1] + N * (k[2] + N * k[0])
+    assert abs(f[peak].item() - N ** 3) <= 1e-11 * N ** 3
+    f[peak] = 0
+    assert torch.linalg.vector_norm(f).item() <= 1e-11 * N ** 3
+    # the backward transform (no scale) of a unit value at bin k is the plane wave itself
+    f.zero_()
+    f[peak] = 1.0
+    z = torch.empty_like(f)
+    plan.backward(f, z)
+    assert rel(z, pw) <= 1e-13
+
+
+CFG3_SIZE = (1024, 1024, 512)
+
+
+def cfg3_terms():
+    """The 16 plane-wave cosines of the real-transform configuration: integer wave vectors
+    uniform in [0, 32)^3 and amplitudes U[0, 1] from numpy seed 2 (same draw order as bench.py)."""
+    rng = np.random.default_rng(2)
+    terms = []
+    for _ in range(16):
+        kv = rng.integers(0, 32, 3)
+        terms.append((tuple(int(v) for v in kv), float(rng.uniform())))
+    return terms
+
+
+def cfg3_field(torch, cuda, dtype, with_noise=True):
+    """sum_m a_m cos(2 pi k_m . r / n) + 1e-3 N(0,1) on 1024 x 1024 x 512 (axis 0 fastest),
+    built in float64 and rounded to dtype; returns (field, sum of squared noise)."""
+    n0, n1, n2 = CFG3_SIZE
+    i = torch.arange(n0, device=cuda, dtype=torch.float64) / n0
+    j = torch.arange(n1, device=cuda, dtype=torch.float64) / n1
+    kk = torch.arange(n2, device=cuda, dtype=torch.float64) / n2
+    x = torch.zeros((n2, n1, n0), device=cuda, dtype=torch.float64)
+    for kv, amp in cfg3_terms():
+        x += amp * torch.cos(2 * math.pi * (kv[2] * kk[:, None, None] + kv[1] * j[None, :, None] +
+                                            kv[0] * i[None, None, :]))
+    noise2 = 0.0
+    if with_noise:
+        g = torch.Generator(device=cuda)
+        g.manual_seed(2)
+        nz = 1e-3 * torch.randn(x.shape, device=cuda, dtype=torch.float64, generator=g)
+        noise2 = (nz * nz).sum().item()
+        x += nz
+        del nz
+    return x.reshape(-1).to(dtype), noise2
+
+
+@pytest.mark.parametrize("rdt", ["float32", "float64"])
+def test_r2c_cfg3_closed_form_spectrum(hmi, cuda, rdt):
+    """r2c along axis 0 of the configuration's plane-wave field: each cosine a cos(2 pi k.r / n)
+    puts a N / 2 at bin k of the half spectrum (513 x 1024 x 512, axis 0 fastest) and, when
+    k_0 = 0, also at its mirror (0, -k_1, -k_2) (k = 0: a N at the origin); everything else is the
+    transform of the noise, whose energy over the half spectrum (planes k_0 = 0 and 512 once, the
+    others twice) is N times the noise energy (Parseval)."""
+    import torch
+    dt = getattr(torch, rdt)
+    cdt = torch.complex64 if rdt == "float32" else torch.complex128
+    n0, n1, n2 = CFG3_SIZE
+    ntot = n0 * n1 * n2
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in CFG3_SIZE])
+    outbox = inbox.r2c(0)
+    plan = hmi.fft3d_r2c(hmi.backend.mi355, inbox, outbox, 0)
+    x, noise2 = cfg3_field(torch, cuda, dt)
+    y = torch.empty(outbox.count(), dtype=cdt, device=cuda)
+    plan.forward(x, y)
+    del x
+    h0 = n0 // 2 + 1
+    expect = {}
+    for kv, amp in cfg3_terms():
+        for sgn in (1, -1):
+            b = ((sgn * kv[0]) % n0, (sgn * kv[1]) % n1, (sgn * kv[2]) % n2)
+            if b[0] < h0:
+                idx = b[0] + h0 * (b[1] + n1 * b[2])
+                expect[idx] = expect.get(idx, 0.0) + amp * ntot / 2
+    # the noise adds about 1e-3 sqrt(N) (~23) to every bin, the float32 rounding far less
+    tol = 2e-6 * ntot
+    for idx, v in expect.items():
+        assert abs(y[idx].item() - v) <= tol, (idx, y[idx].item(), v)
+        y[idx] -= v
+    r = y.view(n2, n1, h0)
+    w = torch.full((h0,), 2.0, dtype=torch.float64, device=cuda)
+    w[0] = w[n0 // 2] = 1.0
+    energy = ((r.real.double() ** 2 + r.imag.double() ** 2) * w).sum().item()
+    assert abs(energy - ntot * noise2) <= 0.01 * ntot * noise2, (energy, ntot * noise2)
+
+
+@pytest.mark.parametrize("rdt", ["float32", "float64"])
+def test_r2c_sine_wave_sign(hmi, cuda, rdt):
+    """sin(2 pi k.r / n) with k_0 > 0: the half spectrum holds -i N / 2 at bin k and nothing else
+    (a conjugated spectrum would hold +i N / 2)."""
+    import torch
+    dt = getattr(torch, rdt)
+    cdt = torch.complex64 if rdt == "float32" else torch.complex128
+    n0, n1, n2 = CFG3_SIZE
+    ntot = n0 * n1 * n2
+    k = (7, 300, 41)
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in CFG3_SIZE])
+    outbox = inbox.r2c(0)
+    plan = hmi.fft3d_r2c(hmi.backend.mi355, inbox, outbox, 0)
+    i = torch.arange(n0, device=cuda, dtype=torch.float64) / n0
+    j = torch.arange(n1, device=cuda, dtype=torch.float64) / n1
+    kk = torch.arange(n2, device=cuda, dtype=torch.float64) / n2
+    x = torch.sin(2 * math.pi * (k[2] * kk[:, None, None] + k[1] * j[None, :, None] + k[0] * i[None, None, :]))
+    x = x.reshape(-1).to(dt)
+    y = torch.empty(outbox.count(), dtype=cdt, device=cuda)
+    plan.forward(x, y)
+    h0 = n0 // 2 + 1
+    idx = k[0] + h0 * (k[1] + n1 * k[2])
+    tol = (2e-6 if rdt == "float32" else 1e-12) * ntot
+    assert abs(y[idx].item() - (-0.5j * ntot)) <= tol, y[idx].item()
+    y[idx] = 0
+    assert torch.linalg.vector_norm(y).item() <= tol * 10
+
+
 def test_r2c_1024x1024x512_float(hmi, cuda):
     """r2c along axis 0 and back (scale::full) and agreement with the double transform."""
     import torch

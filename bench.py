@@ -26,7 +26,7 @@ METRIC = "512³ c2c fwd+bwd ms & %HBM roofline; split-FP16 vs FP32 at ≤5e-6 re
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per pass launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the value the
 # same counter reads for a plain 1 GiB copy (profiles/fft_uniform_512_c64_hbm_traffic.md)
-MEASURED_TRAFFIC = {(512, "complex64"): (2147742898, "profiles/fft_uniform_512_c64_hbm_traffic.md")}
+MEASURED_TRAFFIC = {(512, "complex64"): (2147816896, "profiles/fft_uniform_512_c64_hbm_traffic.md")}
 
 
 def parse():
@@ -51,19 +51,25 @@ def parse():
     return p.parse_args()
 
 
-def pass_times(plan, step, steps, torch, bytes_per_launch):
-    """Average launch duration of each pass kernel over `steps` calls of step(), from the
-    library's per-pass HIP events (recorded on the plan's stream)."""
+def pass_times(plan, step, steps, torch, bytes_per_launch, ms_step):
+    """Average launch duration of each pass kernel in a step.  The library records HIP events on
+    the plan's stream around every pass launch over `steps` further calls of step(); each kernel's
+    share of that event-timed total, applied to ms_step (the time of one step in the timed region,
+    whose launches run back to back with no events between them), gives its average duration
+    there.  The events alone add the few microseconds of each marker to every launch
+    (avg_ms_events)."""
     plan.set_timing(True)
     for _ in range(steps):
         step()
     torch.cuda.synchronize()
     rep = plan.timing()
     plan.set_timing(False)
+    total_all = sum(total for _, total in rep.values())
     out = {}
     for name, (launches, total) in rep.items():
-        avg = total / launches
-        out[name] = {"launches": launches, "total_ms": round(total, 4), "avg_ms": round(avg, 5),
+        avg = ms_step * (total / total_all) / (launches / steps)
+        out[name] = {"launches": launches, "total_ms_events": round(total, 4),
+                     "avg_ms_events": round(total / launches, 5), "avg_ms": round(avg, 5),
                      "GBps": round(bytes_per_launch / (avg * 1e-3) / 1e9, 1),
                      "frac": round(bytes_per_launch / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
@@ -126,8 +132,8 @@ def double_leg(heffte, torch, dev, plan, args, x_np):
     for _ in range(args.warmup):
         step()
     ms = _timed(torch, step, args.steps)
-    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch)
-    dom = max(passes, key=lambda k: passes[k]["total_ms"])
+    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch, ms)
+    dom = max(passes, key=lambda k: passes[k]["total_ms_events"])
     res = {"workload": "c2c 512x512x512 complex<double>, inbox order (0,1,2) -> outbox order (1,2,0), in place, "
                        "caller workspace, forward+backward scale::symmetric, iid N(0,1) (numpy seed 1)",
            "ms_per_pair": round(ms, 4), "gflops_5nlog2n": round(2 * 5.0 * N * math.log2(N) / (ms * 1e-3) / 1e9, 1),
@@ -463,8 +469,8 @@ def main():
     # is the one with the largest total time.
     launches_per_pair = 6
     bytes_per_launch = 2.0 * N * esize  # read + write of the array
-    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch)
-    dominant = max(passes, key=lambda k: passes[k]["total_ms"])
+    passes = pass_times(plan, step, args.steps, torch, bytes_per_launch, ms_pair)
+    dominant = max(passes, key=lambda k: passes[k]["total_ms_events"])
     avg_launch_ms = passes[dominant]["avg_ms"]
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     gflops = 2 * 5.0 * N * math.log2(N) / (ms_pair * 1e-3) / 1e9
@@ -493,10 +499,10 @@ def main():
                      "traffic_source": MEASURED_TRAFFIC.get((n, args.dtype), (None, None))[1],
                      "kernel": dominant, "avg_launch_ms": round(avg_launch_ms, 5),
                      "bytes_per_launch": bytes_per_launch,
-                     "timing": "HIP events around each pass launch on the plan's stream, "
-                               f"{args.steps} pairs after the timed region",
-                     "passes": passes,
-                     "pair_from_passes_ms": round(sum(v["total_ms"] for v in passes.values()) / args.steps, 4)},
+                     "timing": "the timed region's pair time split by each kernel's share of the time "
+                               "between HIP events recorded around each pass launch on the plan's stream "
+                               f"({args.steps} more pairs)",
+                     "passes": passes},
     }
 
     # split-precision MFMA forms of the same transform (HMI_PRECISION_SPLIT_FP16 / _BF16)
@@ -514,8 +520,8 @@ def main():
             e1.record(stream)
             torch.cuda.synchronize()
             ms = e0.elapsed_time(e1) / args.steps
-            fp = pass_times(plan, step, args.steps, torch, bytes_per_launch)
-            dom = max(fp, key=lambda k: fp[k]["total_ms"])
+            fp = pass_times(plan, step, args.steps, torch, bytes_per_launch, ms)
+            dom = max(fp, key=lambda k: fp[k]["total_ms_events"])
             entry = {"ms_per_pair": round(ms, 4), "speedup_vs_fp32": round(ms_pair / ms, 3),
                      "hbm_frac_pair_over_6": round(bytes_per_launch / (ms / launches_per_pair * 1e-3) / 1e9 /
                                                    HBM_PEAK_GBS, 4),

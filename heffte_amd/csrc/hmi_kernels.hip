@@ -125,6 +125,14 @@ __device__ __forceinline__ long long line_offset(unsigned L, const pass_args &a,
                : (long long)c * a.in_sc + (long long)b1 * a.in_sb1 + (long long)b2 * a.in_sb2;
 }
 
+// Index of line L along dimension dim of its numbering (0 column, 1 b1, 2 batch).
+__device__ __forceinline__ unsigned line_index(unsigned L, const pass_args &a, int dim) {
+    const unsigned q = L / (unsigned)a.ncol;
+    if (dim == 0) return L - q * (unsigned)a.ncol;
+    const unsigned b2 = q / (unsigned)a.nb1;
+    return dim == 1 ? q - b2 * (unsigned)a.nb1 : b2;
+}
+
 // One tile (T lines) of an n-point pass; the block has exactly (n / E) * T threads.
 template <typename R, typename RL, int E, int T, int PS, int NTM = 0>
 __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, typename cvec<R>::type *lds) {
@@ -361,6 +369,17 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
         for (int e = 0; e < E; e++)
             if (t2 + P * e < a.store_len)
                 stg<NTS>(&dst[obase + (long long)(t2 + P * e) * a.out_sa], cmk<C>(v[e].x * s, sg * v[e].y * s));
+    } else if (a.tw4) {
+        // first step of a split axis: twiddle w_n4^(j k) before the (conjugating) store
+        C *__restrict__ dst = static_cast<C *>(a.out);
+        const C *__restrict__ w4 = static_cast<const C *>(a.tw4);
+        const unsigned j = line_index(L2, a, a.tw4_dim);
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const unsigned k = t2 + P * e;
+            const C y = cmul(v[e], w4[j * k]);
+            stg<NTS>(&dst[obase + (long long)k * a.out_sa], cmk<C>(y.x * s, sg * y.y * s));
+        }
     } else {
         C *__restrict__ dst = static_cast<C *>(a.out);
 #pragma unroll
@@ -506,7 +525,10 @@ __device__ void generic_tile(const pass_args &a, const generic_plan &g, unsigned
         } else if (a.mode == kModeDCT1B) {
             static_cast<R *>(a.out)[off] = R(2) * y.x * sc;
         } else if (a.out_real) static_cast<R *>(a.out)[off] = y.x * sc;
-        else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
+        else if (a.tw4) {
+            const C z = cmul(y, static_cast<const C *>(a.tw4)[line_index(L, a, a.tw4_dim) * (unsigned)j]);
+            static_cast<C *>(a.out)[off] = cmk<C>(z.x * sc, sg * z.y * sc);
+        } else static_cast<C *>(a.out)[off] = cmk<C>(y.x * sc, sg * y.y * sc);
     }
 }
 
@@ -731,6 +753,8 @@ fft_batched(batch_args b) {
         a.mode = kModeC2C;
         a.stream = 0;
         a.mul = nullptr;
+        a.tw4 = nullptr;
+        a.tw4_dim = 0;
         const double ntot = (double)x.n[0] * (double)x.n[1] * (double)x.n[2];
         a.scale = (b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot));
         batch_tile<R, CLS>(b.kid[x.size_id[ax]], a, b.plans[x.size_id[ax]], it.tile, smem);

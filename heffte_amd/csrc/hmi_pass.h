@@ -59,6 +59,12 @@ struct pass_args {
     const void *mul;  // non-null: every loaded element is first multiplied by the element of this
                       // array at the same offset (same layout and type as in; plain c2c loads only)
     double scale;  // multiplies every stored value (cast to the pass precision)
+    // First step of an axis split in two passes (n4 = n * n2 points, plain c2c store only):
+    // non-null tw4 = table of w_n4^m (m < n4); the value stored at position k of a line is first
+    // multiplied by w_n4^(j k), j = the line's index along dimension tw4_dim (0 column, 1 b1,
+    // 2 batch), which runs over the n2 interleaved sub-sequences.
+    const void *tw4;
+    int tw4_dim;
 };
 
 // Runtime radix plan used by the generic (any length) pass.

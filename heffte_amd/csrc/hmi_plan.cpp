@@ -50,6 +50,27 @@ struct candidate_pass {
     bool identity;
 };
 
+// Factor pair n1 * n2 = n of a length beyond one LDS tile, both at most maxn: pairs whose two
+// lengths have compile-time kernels first, then the most balanced pair.
+bool split_length(int n, int maxn, bool dbl, int &n1, int &n2) {
+    bool found = false;
+    double best = 0.0;
+    for (int d = 2; d <= maxn && d < n; d++) {
+        if (n % d) continue;
+        const int e = n / d;
+        if (e > maxn || e < 2) continue;
+        const int fast = (find_fast_kernel(d, dbl, true) ? 1 : 0) + (find_fast_kernel(e, dbl, true) ? 1 : 0);
+        const double score = fast * 1000.0 - std::fabs(std::log((double)d) - std::log((double)e));
+        if (!found || score > best) {
+            found = true;
+            best = score;
+            n1 = d;
+            n2 = e;
+        }
+    }
+    return found;
+}
+
 }  // namespace
 
 layout layout::ordered(const long long n[3], const int order[3]) {
@@ -90,7 +111,9 @@ fft_plan::fft_plan(kind k, const box3 &inbox, const box3 &outbox, int r2c_dir)
     }
     // workspace: the largest over every variant the plan may run
     long long ws = 0;
-    for_each_variant([&](int dir, bool ir, bool orl, bool ip) { ws = std::max(ws, build(dir, false, ir, orl, ip).ws_elems); });
+    // (a length may be split in one precision only: lines of complex<double> fill LDS sooner)
+    for (bool dbl : {false, true})
+        for_each_variant([&](int dir, bool ir, bool orl, bool ip) { ws = std::max(ws, build(dir, dbl, ir, orl, ip).ws_elems); });
     ws_elems_ = ws;
 }
 
@@ -112,7 +135,14 @@ template <typename F> void fft_plan::for_each_variant(F &&f) const {
 void fft_plan::prepare(bool dbl) {
     for_each_variant([&](int dir, bool ir, bool orl, bool ip) {
         const exec_plan &ep = get_exec_plan(dir, dbl, ir, orl, ip);
-        for (const pass &p : ep.passes) (void)get_twiddles(dbl, p, true);
+        for (const pass &p : ep.passes) {
+            if (p.split) {
+                for (const pass &q : *p.split) (void)get_twiddles(dbl, q, true);
+                (void)get_split_table(dbl, p.n, true);
+            } else {
+                (void)get_twiddles(dbl, p, true);
+            }
+        }
     });
     if (!dbl && precision_ != kPrecPlain && kind_ == kind::c2c) {
         (void)get_mfma_table(precision_, false, true);
@@ -277,7 +307,8 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                     cost += 1.0;
                     if (p.load_colfast && p.store_colfast) cost += 0.15;
                     else if (p.load_colfast || p.store_colfast) cost += 0.05;
-                    if (cp.identity) cost += 0.5;
+                    // a copy pass runs one LDS tile per line: not along an axis too long for one
+                    if (cp.identity) cost += 0.5 + (2 * dims[cp.axis] * (dbl ? 16 : 8) > kGenericMaxLds ? 100.0 : 0.0);
                     ep.passes.push_back(p);
                     cur = p.lout;
                     cur_buf = p.dst;
@@ -315,6 +346,14 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
     if (!std::isfinite(best_cost)) throw std::runtime_error("no valid pass schedule for these boxes");
 
     for (pass &p : best.passes) configure(p, dbl);
+    // the step buffer of split axes follows the intermediate arrays in the workspace
+    long long split_elems = 0;
+    for (pass &p : best.passes)
+        if (p.split) {
+            p.split_ws = best.ws_elems;
+            split_elems = std::max(split_elems, p.lin.n[0] * p.lin.n[1] * p.lin.n[2]);
+        }
+    best.ws_elems += split_elems;
     return best;
 }
 
@@ -364,8 +403,32 @@ void fft_plan::configure(pass &p, bool dbl) const {
     }
     g.postoff = off;
     const size_t per_line = 2 * (size_t)p.n * elem;
-    if (per_line > (size_t)kGenericMaxLds)
-        throw std::runtime_error("transform length " + std::to_string(p.n) + " exceeds the single-pass LDS capacity");
+    if (per_line > (size_t)kGenericMaxLds) {
+        // beyond one LDS tile: two passes of lengths n1 * n2 = n (plain c2c lines only)
+        int n1 = 0, n2 = 0;
+        if (p.mode != kModeC2C || p.identity || p.store_len != p.n || p.herm_load ||
+            !split_length(p.n, (int)(kGenericMaxLds / (2 * elem)), dbl, n1, n2))
+            throw std::runtime_error("transform length " + std::to_string(p.n) +
+                                     " exceeds the single-pass LDS capacity and has no two-pass split");
+        auto steps = std::make_shared<std::array<pass, 2>>();
+        for (int i = 0; i < 2; i++) {
+            pass &q = (*steps)[i];
+            q.n = q.store_len = (i == 0) ? n1 : n2;
+            q.mode = kModeC2C;
+            q.load_colfast = (i == 0) ? 1 : 0;
+            q.store_colfast = 1;
+            const long long big = 1LL << 20;  // many columns: no cap on the lines per block
+            q.lin = q.lout = layout{{big, big, big}, {1, big, big * big}};
+            q.col = 0;
+            configure(q, dbl);
+        }
+        p.split_n1 = n1;
+        p.split_n2 = n2;
+        p.split = steps;
+        p.fast = nullptr;
+        p.lines = 1;
+        return;
+    }
     int lines = (int)std::min<size_t>(64, std::max<size_t>(1, (64 * 1024) / per_line));
     // no more lines per block than columns
     const long long ncol = p.lin.n[p.col] > 0 ? std::max(p.lin.n[p.col], p.lout.n[p.col]) : 1;
@@ -485,7 +548,10 @@ namespace {
 std::string pass_kernel_name(const pass &p, bool dbl, bool mfma, int precision, const pass_args &a) {
     const char *R = dbl ? "double" : "float";
     std::string s;
-    if (mfma) {
+    if (p.split) {
+        s = std::string("split ") + R + " " + std::to_string(p.split_n1) + " x " + std::to_string(p.split_n2) +
+            " (two launches per transform)";
+    } else if (mfma) {
         s = std::string("fft512_mfma<") + (precision == kSplitF16 ? "kSplitF16" : "kSplitBF16") + ", " +
             (a.load_colfast ? "true" : "false") + ", " + (a.store_colfast ? "true" : "false") + ">";
     } else if (p.fast) {
@@ -541,6 +607,173 @@ const twiddles &fft_plan::get_mfma_table(int prec, bool conj, bool may_alloc) {
     return mt;
 }
 
+const twiddles &fft_plan::get_split_table(bool dbl, int n, bool may_alloc) {
+    // w_n^m, m < n, of a split axis (key: bit 7 set, out of the range of the pass keys' mode bits)
+    const long long key = ((long long)n << 8) | (dbl ? 1 : 0) | 0x80;
+    std::lock_guard<std::mutex> lock(mtx_);
+    auto it = tw_.find(key);
+    if (it != tw_.end()) return it->second;
+    if (!may_alloc) throw std::runtime_error(kNotResident);
+    twiddles t;
+    if (dbl) {
+        std::vector<double> v = make_table<double>(nullptr, 0, false, n, n);
+        t.bytes = v.size() * sizeof(double);
+        t.dev = upload_bytes(v.data(), t.bytes);
+    } else {
+        std::vector<float> v = make_table<float>(nullptr, 0, false, n, n);
+        t.bytes = v.size() * sizeof(float);
+        t.dev = upload_bytes(v.data(), t.bytes);
+    }
+    return tw_.emplace(key, t).first->second;
+}
+
+pass_args pass_launch_args(kind k, const exec_plan &ep, const pass &p, int dir, bool dbl, int batch, const void *in,
+                           void *out, void *ws, const void *tw, double scale, const void *mul) {
+    pass_args a{};
+    auto bufptr = [&](buf b) -> void * {
+        return b == buf::in ? const_cast<void *>(in) : (b == buf::out ? out : ws);
+    };
+    a.in = bufptr(p.src);
+    a.out = bufptr(p.dst);
+    a.tw = tw;
+    a.in_sa = p.lin.stride[p.axis];
+    a.in_sc = p.lin.stride[p.col];
+    a.in_sb1 = p.lin.stride[p.b1];
+    a.in_sb2 = p.lin.n[0] * p.lin.n[1] * p.lin.n[2];
+    a.out_sa = p.lout.stride[p.axis];
+    a.out_sc = p.lout.stride[p.col];
+    a.out_sb1 = p.lout.stride[p.b1];
+    a.out_sb2 = p.lout.n[0] * p.lout.n[1] * p.lout.n[2];
+    a.ncol = (int)p.lout.n[p.col];
+    a.nb1 = (int)p.lout.n[p.b1];
+    a.nbatch = batch;
+    if ((long long)a.ncol * a.nb1 * batch > 0x7fffffffLL) throw std::runtime_error("too many lines in one pass");
+    a.nlines = a.ncol * a.nb1 * batch;
+    a.load_colfast = p.load_colfast;
+    a.store_colfast = p.store_colfast;
+    // r2r modes run their inverse through explicit conjugation inside the kernel
+    a.conj = (dir > 0 && p.mode < kModeDCT2) ? 1 : 0;
+    a.in_real = p.in_real;
+    a.out_real = p.out_real;
+    a.store_len = p.store_len;
+    a.herm_load = p.herm_load;
+    a.mode = p.mode;
+    // the output of a pass that fits the 256 MB Infinity Cache can serve the next pass from
+    // there; larger ones stream past the caches: non-temporal stores, and non-temporal loads
+    // when a side walks the column axis (measured on the 512^3 complex<float> passes: the
+    // contiguous pass 365 -> 353 us with plain loads, the column-chunk pass 388 -> 413 us)
+    const long long out_bytes = (long long)batch * p.lout.n[0] * p.lout.n[1] * p.lout.n[2] *
+                                (p.out_real ? (dbl ? 8 : 4) : (dbl ? 16 : 8));
+    a.stream = out_bytes <= kStreamBytes ? 0
+               : (p.load_colfast || p.store_colfast) ? kStreamLoadsStores : kStreamStores;
+    a.scale = p.last ? scale : 1.0;
+    a.mul = nullptr;
+    if (mul && &p == &ep.passes.front()) {
+        // the fused product needs the plain complex load of the first pass
+        if (k != kind::c2c || p.in_real || p.herm_load || p.mode != kModeC2C || p.src != buf::in)
+            throw std::runtime_error("internal error: the product cannot be fused into this pass");
+        a.mul = mul;
+    }
+    return a;
+}
+
+void split_steps(const pass &p, const pass_args &base, bool dbl, int bt, void *tmp, pass_args step[2]) {
+    const int ax = p.axis, n1 = p.split_n1, n2 = p.split_n2;
+    const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+    const size_t elem = dbl ? 16 : 8;
+    const long long cnt_in = p.lin.n[0] * p.lin.n[1] * p.lin.n[2];
+    const long long cnt_out = p.lout.n[0] * p.lout.n[1] * p.lout.n[2];
+    // the line-index dimensions of a step: extent, stride on the load side, stride on the store side
+    struct dim {
+        long long n, si, so;
+        int id;  // 0: the interleaved index (j2 of step 1, k1 of step 2), 1 and 2: the other axes
+    };
+    auto order = [](std::array<dim, 3> d, bool by_load) {
+        // the fastest dimension (on the side that walks the column axis) is the column
+        std::stable_sort(d.begin(), d.end(), [&](const dim &x, const dim &y) {
+            const long long sx = x.n > 1 ? (by_load ? x.si : x.so) : (1LL << 62);
+            const long long sy = y.n > 1 ? (by_load ? y.si : y.so) : (1LL << 62);
+            return sx < sy;
+        });
+        return d;
+    };
+    const long long sa_in = p.lin.stride[ax], sa_out = p.lout.stride[ax];
+    const char *src = static_cast<const char *>(base.in) + (size_t)bt * cnt_in * (p.in_real ? elem / 2 : elem);
+    char *buf = static_cast<char *>(tmp) + (size_t)bt * cnt_in * elem;
+    char *dst = static_cast<char *>(base.out) + (size_t)bt * cnt_out * (p.out_real ? elem / 2 : elem);
+    auto fill = [](pass_args &a, const std::array<dim, 3> &d) {
+        a.in_sc = d[0].si;
+        a.out_sc = d[0].so;
+        a.in_sb1 = d[1].si;
+        a.out_sb1 = d[1].so;
+        a.in_sb2 = d[2].si;
+        a.out_sb2 = d[2].so;
+        a.ncol = (int)d[0].n;
+        a.nb1 = (int)d[1].n;
+        a.nbatch = (int)d[2].n;
+        a.nlines = (int)(d[0].n * d[1].n * d[2].n);
+    };
+    // step 1: n1-point lines of stride n2 sa_in, one per (j2, o1, o2), into buf (layout lin)
+    {
+        const std::array<dim, 3> d = order({dim{n2, sa_in, sa_in, 0}, dim{p.lin.n[o1], p.lin.stride[o1], p.lin.stride[o1], 1},
+                                            dim{p.lin.n[o2], p.lin.stride[o2], p.lin.stride[o2], 2}},
+                                           true);
+        pass_args &a = step[0];
+        a = base;
+        a.in = src;
+        a.out = buf;
+        a.mul = base.mul ? static_cast<const char *>(base.mul) + (size_t)bt * cnt_in * elem : nullptr;
+        a.in_sa = a.out_sa = (long long)n2 * sa_in;
+        fill(a, d);
+        a.load_colfast = a.store_colfast = 1;
+        a.out_real = 0;
+        a.store_len = n1;
+        a.mode = kModeC2C;
+        a.scale = 1.0;
+        a.tw4_dim = d[0].id == 0 ? 0 : (d[1].id == 0 ? 1 : 2);
+    }
+    // step 2: n2-point lines of buf (stride sa_in), one per (k1, o1, o2), to k1 + n1 k2 of dst
+    {
+        const bool lcf = p.lin.fastest() != ax;
+        const std::array<dim, 3> d = order({dim{n1, (long long)n2 * sa_in, sa_out, 0},
+                                            dim{p.lin.n[o1], p.lin.stride[o1], p.lout.stride[o1], 1},
+                                            dim{p.lin.n[o2], p.lin.stride[o2], p.lout.stride[o2], 2}},
+                                           lcf);
+        pass_args &a = step[1];
+        a = base;
+        a.in = buf;
+        a.out = dst;
+        a.mul = nullptr;
+        a.in_real = 0;
+        a.in_sa = sa_in;
+        a.out_sa = (long long)n1 * sa_out;
+        fill(a, d);
+        a.load_colfast = lcf ? 1 : 0;
+        a.store_colfast = 1;
+        a.store_len = n2;
+        a.mode = kModeC2C;
+        a.tw4 = nullptr;
+        a.tw4_dim = 0;
+    }
+}
+
+void fft_plan::launch_split(const pass &p, const pass_args &base, bool dbl, int batch, void *tmp) {
+    const size_t elem = dbl ? 16 : 8;
+    for (int bt = 0; bt < batch; bt++) {
+        pass_args st[2];
+        split_steps(p, base, dbl, bt, tmp, st);
+        st[0].tw4 = get_split_table(dbl, p.n, false).dev;
+        for (int i = 0; i < 2; i++) {
+            const pass &q = (*p.split)[i];
+            st[i].tw = get_twiddles(dbl, q, false).dev;
+            const long long nblocks = ((long long)st[i].nlines + q.lines - 1) / q.lines;
+            if (q.fast) check_hip(q.fast->launch(st[i], nblocks, stream_), "split FFT pass");
+            else check_hip(launch_generic(dbl, st[i], q.gen, nblocks, 2 * (size_t)q.gen.lines * q.n * elem, stream_),
+                           "split FFT pass");
+        }
+    }
+}
+
 void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch, const void *in, void *out, void *ws,
                        int scaling, const void *mul, bool no_alloc) {
     if (batch < 1) throw std::invalid_argument("batch size must be positive");
@@ -572,7 +805,13 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         const pass &p = ep.passes[i];
         const bool mfma = split && p.n == 512 && p.mode == kModeC2C && !p.identity && !p.in_real && !p.out_real &&
                           !p.herm_load && p.store_len == p.n && !(mul && i == 0);
-        tables[i] = mfma ? get_mfma_table(precision_, dir > 0, may_alloc).dev : get_twiddles(dbl, p, may_alloc).dev;
+        if (p.split) {
+            // both steps' tables and the step twiddles, looked up now (launch_split uses them)
+            for (const pass &q : *p.split) (void)get_twiddles(dbl, q, may_alloc);
+            tables[i] = get_split_table(dbl, p.n, may_alloc).dev;
+        } else {
+            tables[i] = mfma ? get_mfma_table(precision_, dir > 0, may_alloc).dev : get_twiddles(dbl, p, may_alloc).dev;
+        }
     }
     void *tmp = nullptr;
     if (ep.ws_elems > 0 && !ws) {
@@ -606,6 +845,8 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
         // every element a pass touches lies inside the buffer it names
         if (extent(p.lin) > capacity(p.src) || extent(p.lout) > capacity(p.dst))
             throw std::runtime_error("internal error: pass exceeds its buffer");
+        if (p.split && batch * (p.split_ws + p.lin.n[0] * p.lin.n[1] * p.lin.n[2]) > cap_ws)
+            throw std::runtime_error("internal error: split pass exceeds the workspace");
         if (debug)
             std::fprintf(stderr,
                          "[hmi] pass axis=%d n=%d src=%d dst=%d col=%d b1=%d lcf=%d scf=%d real=%d/%d herm=%d "
@@ -613,55 +854,14 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
                          p.axis, p.n, (int)p.src, (int)p.dst, p.col, p.b1, p.load_colfast, p.store_colfast,
                          p.in_real, p.out_real, p.herm_load, p.store_len, p.fast ? 1 : 0, p.lines, p.lin.stride[0],
                          p.lin.stride[1], p.lin.stride[2], p.lout.stride[0], p.lout.stride[1], p.lout.stride[2]);
-        pass_args a{};
-        auto bufptr = [&](buf b) -> void * {
-            return b == buf::in ? const_cast<void *>(in) : (b == buf::out ? out : ws);
-        };
-        a.in = bufptr(p.src);
-        a.out = bufptr(p.dst);
-        a.tw = tables[&p - &ep.passes.front()];
-        a.in_sa = p.lin.stride[p.axis];
-        a.in_sc = p.lin.stride[p.col];
-        a.in_sb1 = p.lin.stride[p.b1];
-        a.in_sb2 = p.lin.n[0] * p.lin.n[1] * p.lin.n[2];
-        a.out_sa = p.lout.stride[p.axis];
-        a.out_sc = p.lout.stride[p.col];
-        a.out_sb1 = p.lout.stride[p.b1];
-        a.out_sb2 = p.lout.n[0] * p.lout.n[1] * p.lout.n[2];
-        a.ncol = (int)p.lout.n[p.col];
-        a.nb1 = (int)p.lout.n[p.b1];
-        a.nbatch = batch;
-        if ((long long)a.ncol * a.nb1 * batch > 0x7fffffffLL) throw std::runtime_error("too many lines in one pass");
-        a.nlines = a.ncol * a.nb1 * batch;
-        a.load_colfast = p.load_colfast;
-        a.store_colfast = p.store_colfast;
-        // r2r modes run their inverse through explicit conjugation inside the kernel
-        a.conj = (dir > 0 && p.mode < kModeDCT2) ? 1 : 0;
-        a.in_real = p.in_real;
-        a.out_real = p.out_real;
-        a.store_len = p.store_len;
-        a.herm_load = p.herm_load;
-        a.mode = p.mode;
-        // the output of a pass that fits the 256 MB Infinity Cache can serve the next pass from
-        // there; larger ones stream past the caches: non-temporal stores, and non-temporal loads
-        // when a side walks the column axis (measured on the 512^3 complex<float> passes: the
-        // contiguous pass 365 -> 353 us with plain loads, the column-chunk pass 388 -> 413 us)
-        const long long out_bytes = (long long)batch * p.lout.n[0] * p.lout.n[1] * p.lout.n[2] *
-                                    (p.out_real ? (dbl ? 8 : 4) : (dbl ? 16 : 8));
-        a.stream = out_bytes <= kStreamBytes ? 0
-                   : (p.load_colfast || p.store_colfast) ? kStreamLoadsStores : kStreamStores;
-        a.scale = p.last ? scale : 1.0;
-        a.mul = nullptr;
-        if (mul && &p == &ep.passes.front()) {
-            // the fused product needs the plain complex load of the first pass
-            if (kind_ != kind::c2c || p.in_real || p.herm_load || p.mode != kModeC2C || p.src != buf::in)
-                throw std::runtime_error("internal error: the product cannot be fused into this pass");
-            a.mul = mul;
-        }
+        pass_args a = pass_launch_args(kind_, ep, p, dir, dbl, batch, in, out, ws, tables[&p - &ep.passes.front()],
+                                       scale, mul);
         const long long nblocks = ((long long)a.nlines + p.lines - 1) / p.lines;
         const bool mfma = split && p.n == 512 && p.mode == kModeC2C && !p.identity && !p.in_real && !p.out_real &&
                           !p.herm_load && p.store_len == p.n && !a.mul;
-        if (mfma) {
+        if (p.split) {
+            launch_split(p, a, dbl, batch, static_cast<char *>(ws) + (size_t)batch * p.split_ws * (dbl ? 16 : 8));
+        } else if (mfma) {
             // the split-precision form: a.tw holds W512 and the split W32 / W16 operand tables
             // of the direction, 16-line tiles; the backward pass multiplies by the conjugate
             // matrices directly (no conjugation of the data on load and store)

@@ -63,6 +63,15 @@ struct pass {
     const fast_kernel_info *fast = nullptr;
     generic_plan gen{};
     int lines = 1;  // lines (columns) per block
+    // A length whose line does not fit one LDS tile runs as two passes (four-step), n = n1 n2,
+    // position j = n2 j1 + j2, bin k = k1 + n1 k2: the first step takes the n1-point lines of
+    // stride n2 (one per j2) into a buffer of the pass's input layout at position n2 k1 + j2,
+    // each value twiddled by w_n^(j2 k1); the second takes the n2-point lines (one per k1) of that
+    // buffer to the destination at k1 + n1 k2.  split holds the kernel configuration of the two
+    // steps; the buffer lies in the workspace at split_ws (elements per transform).
+    int split_n1 = 0, split_n2 = 0;
+    std::shared_ptr<const std::array<pass, 2>> split;
+    long long split_ws = 0;
 };
 
 struct exec_plan {
@@ -132,6 +141,8 @@ class fft_plan {
     exec_plan build(int dir, bool dbl, bool in_real, bool out_real, bool in_place) const;
     void configure(pass &p, bool dbl) const;
     const twiddles &get_twiddles(bool dbl, const pass &p, bool may_alloc);
+    const twiddles &get_split_table(bool dbl, int n, bool may_alloc);
+    void launch_split(const pass &p, const pass_args &base, bool dbl, int batch, void *tmp);
     const twiddles &get_mfma_table(int prec, bool conj, bool may_alloc);
     template <typename F> void for_each_variant(F &&f) const;
 
@@ -165,6 +176,13 @@ std::vector<double> generic_table_f64(const generic_plan &g);
 // Stage twiddles of a compile-time radix plan (no root or real-data tables).
 std::vector<float> stage_table_f32(const int *radix, int nstages);
 std::vector<double> stage_table_f64(const int *radix, int nstages);
+
+// Launch arguments of pass p of plan ep (the split steps derive theirs from these).
+pass_args pass_launch_args(kind k, const exec_plan &ep, const pass &p, int dir, bool dbl, int batch, const void *in,
+                           void *out, void *ws, const void *tw, double scale, const void *mul);
+// Launch arguments of the two steps of a split pass for transform bt of a batch (tables not
+// set): base = the arguments the pass would have unsplit, tmp = the step buffer.
+void split_steps(const pass &p, const pass_args &base, bool dbl, int bt, void *tmp, pass_args step[2]);
 
 // Permutation of a box between two orders (reshape3d_transpose semantics).
 void permute(int elem_bytes, const box3 &inbox, const int out_order[3], int batch, const void *in, void *out,

@@ -147,12 +147,21 @@ def test_proc_setup_and_split_world(hmi):
     assert len(covered) == w2.count()
 
 
-def test_length_beyond_single_pass_capacity_is_rejected(hmi):
-    """A line that does not fit the 160 KiB LDS of one workgroup (2 n element sizes) is refused
-    at plan creation with a message, not run."""
+def test_length_beyond_single_pass_capacity(hmi):
+    """A line that does not fit the 160 KiB LDS of one workgroup (2 n element sizes) runs as two
+    passes n = n1 n2, with a step buffer of one array in the workspace; a length with no such
+    split (a prime beyond the capacity) is refused at plan creation with a message, not run."""
     ok = hmi.box3d([0, 0, 0], [10239, 0, 0])
     hmi.fft3d(hmi.backend.mi355, ok, ok)
-    big = hmi.box3d([0, 0, 0], [10240 + 1, 0, 0])  # 10242 = 2 * 3 * 3 * 569
+    n = 16384 * 4 * 2
+    split = hmi.box3d([0, 0, 0], [16383, 3, 1])  # 16384 = 128 x 128
+    p = hmi.fft3d(hmi.backend.mi355, split, split)
+    # the step buffer, and an intermediate array for the in-place call with real input
+    assert p.size_workspace() == 2 * n
+    permuted = hmi.box3d([0, 0, 0], [16383, 3, 1], [2, 1, 0])
+    p = hmi.fft3d(hmi.backend.mi355, split, permuted)
+    assert p.size_workspace() == 2 * n  # an intermediate array and the step buffer
+    big = hmi.box3d([0, 0, 0], [10246, 0, 0])  # 10247 is prime
     with pytest.raises(hmi.heffte_input_error) as e:
         hmi.fft3d(hmi.backend.mi355, big, big)
     assert "LDS" in str(e.value)

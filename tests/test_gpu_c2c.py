@@ -210,3 +210,61 @@ def test_pass_timing_report(hmi, cuda, oracle):
     plan.set_timing(False)
     plan.forward_buffered(tx, ty, ws)
     assert plan.timing() == {}
+
+
+@pytest.mark.parametrize("dtype,size,orders", [
+    ("complex64", (16384, 4, 2), ((0, 1, 2), (0, 1, 2))),
+    ("complex64", (16384, 4, 2), ((0, 1, 2), (2, 1, 0))),
+    ("complex64", (3, 12288, 2), ((1, 2, 0), (0, 1, 2))),
+    ("complex64", (11025, 2, 2), ((0, 1, 2), (1, 0, 2))),
+    ("complex128", (8192, 3, 2), ((0, 1, 2), (0, 1, 2))),
+    ("complex128", (2, 3, 8192), ((2, 0, 1), (1, 2, 0))),
+    ("complex128", (5376, 2, 2), ((0, 1, 2), (2, 0, 1))),
+])
+def test_split_lengths(hmi, cuda, oracle, dtype, size, orders):
+    """Lengths beyond one LDS tile (complex<float> > 10240, complex<double> > 5120) run as two
+    passes, n = n1 n2 (16384 = 128 x 128, 12288 = 96 x 128, 11025 = 105 x 105 on runtime-plan
+    steps, 8192 = 64 x 128, 5376 = 42 x 128): forward against the oracle out of and in place, a
+    batch of 2, and the backward transform with scale::full."""
+    inorder, outorder = orders
+    tol = fc.TOL[dtype]
+    for in_place in (False, True):
+        x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=12, in_place=in_place)
+        ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
+        assert fc.rel_l2(y, ref) <= tol, (in_place, fc.rel_l2(y, ref))
+    n = int(np.prod(size))
+    # a plan from outorder to inorder, run backward: data in inorder -> outorder
+    x, y, _ = run_c2c(hmi, cuda, size, outorder, inorder, dtype, seed=13, batch=2, scaling=1, direction="backward")
+    ref = oracle_c2c(oracle, size, inorder, outorder, x, +1, 1.0 / n, batch=2)
+    assert fc.rel_l2(y, ref) <= tol
+
+
+def test_split_length_real_input_and_r2c(hmi, cuda, oracle):
+    """A split axis behind the real-input overload of fft3d and as a complex axis of fft3d_r2c."""
+    import torch
+    size = (16384, 3, 2)
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (0, 1, 2))
+    outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (2, 1, 0))
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    xr = np.random.default_rng(14).uniform(-1, 1, n)
+    t_out = torch.empty(n, dtype=torch.complex64, device=cuda)
+    plan.forward(torch.from_numpy(xr.astype(np.float32)).to(cuda), t_out)
+    bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size], (0, 1, 2))
+    bo = oracle.Box.make((0, 0, 0), [s - 1 for s in size], (2, 1, 0))
+    assert fc.rel_l2(fc.to_host(t_out), oracle.fft3d_real_c2c(bi, bo, xr)) <= 5e-6
+    # r2c along axis 1 of a box whose axis 0 is split
+    size = (12288, 6, 2)
+    rin = hmi.box3d([0, 0, 0], [s - 1 for s in size])
+    rout = rin.r2c(1)
+    rplan = hmi.fft3d_r2c(hmi.backend.mi355, rin, rout, 1)
+    x = np.random.default_rng(15).standard_normal(rin.count())
+    tx = torch.from_numpy(x).to(cuda)
+    ty = torch.empty(rout.count(), dtype=torch.complex128, device=cuda)
+    rplan.forward(tx, ty)
+    ref = oracle.fft3d_r2c(oracle.Box.make((0, 0, 0), [s - 1 for s in size]),
+                           oracle.Box.make(rout.low, rout.high), 1, x)
+    assert fc.rel_l2(fc.to_host(ty), ref) <= 1e-13
+    back = torch.empty_like(tx)
+    rplan.backward(ty, back, hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(back), x) <= 1e-13

@@ -56,6 +56,20 @@ const char *kernels_build_stamp();
 // work items of one register class.
 hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
 
+// Fused x-y phase of a batch: each work item loads whole x-y planes (contiguous) of one
+// transform into LDS, runs the x-axis and y-axis FFTs there and stores the planes back, so the x
+// and y axes cost one read and one write of the array instead of two each.  Transforms whose x
+// and y lengths both have register tiles (batch_xy_supported) take it.  Two builds: 512 threads
+// with planes of at most kXySmallBudget elements (one per register class of tile, cls = the
+// larger class of the two lengths) and 1024 threads for planes up to kXyLargeBudget elements
+// (tiles of the first two classes only).
+constexpr int kXySmallThreads = 512, kXyLargeThreads = 1024;
+constexpr int kXySmallBudget[2] = {8192, 4096};    // elements: 64 KiB of LDS (complex<float>, <double>)
+constexpr int kXyLargeBudget[2] = {20480, 10240};  // elements: 160 KiB of LDS
+bool batch_xy_supported(int kid);
+hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
+                             hipStream_t s);
+
 // Precision forms of the c2c passes: plain FP32/FP64 vector ALU, or the 512-point
 // complex<float> pass on the matrix cores with split-precision operands.
 enum : int { kPrecPlain = 0, kSplitF16 = 1, kSplitBF16 = 2 };

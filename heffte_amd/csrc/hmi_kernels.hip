@@ -46,24 +46,24 @@ template <int... Rs> struct radices {
 template <int PS> __device__ __forceinline__ int padj(int j) { return j + (j >> PS); }
 
 // LDS tile of the uniform pass: element j of line l.  Rows of n elements padded by one element
-// per 2^ps (+1), or, for the 512-point tiles with 8 elements per thread, rows of n + 2 elements
-// with the low bits of j XOR-swizzled by its higher bits.  LDS serves 8-byte accesses 32 lanes
-// per cycle over its 64 four-byte banks (16-byte accesses 16 lanes): the swizzle gives every
-// exchange of those tiles distinct banks -- the stage writes (one line per wave: positions
-// 8t + q and 64a + b + 8q), the reads of positions t + 64e in either thread mapping, and the
-// column-walking mapping, where the 16 lines of a half-wave differ by 2 rows of 32 slots.
+// per 2^ps (+1), or, for the 512-point complex<float> tiles (8 elements per thread), rows of
+// n + 32 elements with the low 5 bits of j mapped linearly (j0 -> 0x11, j1..j3 -> themselves, j4 -> 0x12),
+// XOR-ed with 0x1 * j5 and 0xC * j6 and with the line's low 4 bits.  Measured on gfx950 (a probe
+// of strides and row patterns under rocprofv3's SQ_LDS_BANK_CONFLICT): ds_read_b64 serves 32
+// lanes per pass over 64 four-byte banks, ds_write_b64 16 lanes per pass over 32 banks.  Under
+// that rule every exchange of these tiles is conflict-free: stage writes of one line per wave
+// (positions 8t + q and 64a + b + 8q), reads of positions t + 64e, and, in the mapping where
+// consecutive lanes walk 16 lines, writes of 16 lines at one position and reads of 16 lines at
+// two adjacent positions (the padded rows keep 1 conflict cycle per read there).
 template <int N, int E, int PS, int BYTES> struct lds_map {
-    static constexpr bool swz = (N == 512 && E == 8);
-    static constexpr int RS = swz ? N + 2 : N + (N >> PS) + 1;
+    static constexpr bool swz = (N == 512 && E == 8 && BYTES == 8);
+    // (swizzled rows keep 32 more slots: the c2r load parks position N there)
+    static constexpr int RS = swz ? N + 32 : N + (N >> PS) + 1;
     __device__ static __forceinline__ int idx(int l, int j) {
         if constexpr (swz) {
-            if constexpr (BYTES == 8) {
-                const int h = j >> 5;
-                return l * RS + (j ^ ((h & 7) | (((h >> 1) & 3) << 3)));
-            } else {
-                const int h = j >> 4;
-                return l * RS + (j ^ ((h & 7) | (((h >> 2) & 1) << 3)));
-            }
+            const int lo = ((j & 0xE) ^ ((j & 1) * 0x11) ^ (((j >> 4) & 1) * 0x12)) ^ ((j >> 5) & 1) ^
+                           (((j >> 6) & 1) * 0xC) ^ (l & 15);
+            return l * RS + ((j & ~31) | lo);
         } else {
             return l * RS + padj<PS>(j);
         }
@@ -783,6 +783,193 @@ hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, s
     if (cls < 0 || cls >= kBatchClasses) return hipErrorInvalidValue;
     const batched_fn fn = batched_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
     fn<<<dim3((unsigned)nblocks), dim3(dbl ? 128 : 256), lds_bytes, s>>>(b);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ batched, fused x-y
+
+// Stockham stages of one n-point line held in LDS at lb[j * es], E elements per thread (P = n / E
+// threads per line); each exchange goes through the line's own slots.  Every thread of the block
+// calls it (the barriers are block-wide); threads without a line (valid = false) only take part
+// in the barriers.
+template <int S, typename RL, int E, int P, typename C>
+__device__ __forceinline__ void lds_line_stages(C (&v)[E], C *lb, int es, const C *__restrict__ tw, int t,
+                                                bool valid) {
+    constexpr int R = RL::at(S);
+    constexpr int Ls = RL::ls(S);
+    constexpr int NB = E / R;
+    constexpr bool last = (S + 1 == RL::count);
+    constexpr int TWO = RL::twoff(S);
+    static_assert(E % R == 0, "every radix must divide the elements per thread");
+#pragma unroll
+    for (int r = 0; r < NB; r++) {
+        const int b = t + P * r;
+        const int k = b % Ls;
+        C x[R];
+#pragma unroll
+        for (int p = 0; p < R; p++) x[p] = v[r + NB * p];
+        if constexpr (Ls > 1) {
+            const C *w = tw + TWO + k * (R - 1);
+#pragma unroll
+            for (int p = 1; p < R; p++) x[p] = cmul(x[p], w[p - 1]);
+        }
+        dft<R>::run(x);
+        if constexpr (last) {
+#pragma unroll
+            for (int q = 0; q < R; q++) v[r + NB * q] = x[q];
+        } else if (valid) {
+            const int base = (b / Ls) * Ls * R + k;
+#pragma unroll
+            for (int q = 0; q < R; q++) lb[(base + q * Ls) * es] = x[q];
+        }
+    }
+    if constexpr (!last) {
+        __syncthreads();
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = lb[(t + P * e) * es];
+        }
+        __syncthreads();
+        lds_line_stages<S + 1, RL, E, P>(v, lb, es, tw, t, valid);
+    }
+}
+
+// n-point FFTs of nlines lines held in LDS (line l at buf + base_of(l), element stride es), by
+// NTH threads in rounds of NTH / P lines.  ACROSS = false: consecutive lanes walk a line (lines
+// whose elements are adjacent); true: consecutive lanes walk consecutive lines (lines whose
+// starts are adjacent), so the LDS accesses of a wave stay on consecutive slots either way.
+template <typename RL, int E, int NTH, bool ACROSS, typename C, typename BaseF>
+__device__ __forceinline__ void lds_lines_fft(C *buf, int nlines, int es, const C *__restrict__ tw, BaseF base_of) {
+    constexpr int N = RL::n(), P = N / E, LPI = NTH / P;
+    static_assert(LPI >= 1, "a line needs at most the block's threads");
+    const int tid = threadIdx.x;
+    const int l0 = ACROSS ? tid % LPI : tid / P;
+    const int t = ACROSS ? tid / LPI : tid % P;
+    const bool inb = tid < LPI * P;
+    for (int first = 0; first < nlines; first += LPI) {
+        const int line = first + l0;
+        const bool valid = inb && line < nlines;
+        C *lb = buf + (valid ? base_of(line) : 0);
+        C v[E];
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = lb[(t + P * e) * es];
+        }
+        __syncthreads();
+        lds_line_stages<0, RL, E, P>(v, lb, es, tw, t, valid);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < E; e++) lb[(t + P * e) * es] = v[e];
+        }
+        __syncthreads();
+    }
+}
+
+// Register tiles the fused phase can run: compile-time tiles with E > 0 (not the staged tile)
+#define HMI_XY_OK(K, N, E, T, PS, ...) (E > 0),
+static const bool kXyOk[] = {HMI_BATCH_TABLE(HMI_XY_OK)};
+bool batch_xy_supported(int kid) { return kid >= 0 && kXyOk[kid]; }
+
+// the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
+template <typename R, int CLS, int NTH, bool ACROSS, typename BaseF>
+__device__ __forceinline__ void xy_axis(int kid, typename cvec<R>::type *buf, int nlines, int es,
+                                        const typename cvec<R>::type *tw, BaseF base_of) {
+#define HMI_XY_CASE(K, N, E, T, PS, ...)                                                        \
+    case K:                                                                                     \
+        if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1) \
+            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(buf, nlines, es, tw, base_of); \
+        break;
+    switch (kid) {
+        HMI_BATCH_TABLE(HMI_XY_CASE)
+        default: break;
+    }
+#undef HMI_XY_CASE
+}
+
+// Fused x-y phase: work item = (transform, group of G consecutive z-planes), G = max(1, budget /
+// (n0 n1)).  The planes are contiguous in memory: one coalesced load into LDS (conjugated for the
+// backward transform), the n0-point FFTs of the G n1 rows, the n1-point FFTs of the G n0
+// columns, one coalesced store (conjugated back).  No scale: the z phase applies it.
+template <typename R, int CLS, int NTH>
+__device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &it, typename cvec<R>::type *buf) {
+    using C = typename cvec<R>::type;
+    const batch_xform x = b.xforms[it.xform];
+    const int n0 = x.n[0], n1 = x.n[1], n2 = x.n[2];
+    const int plane = n0 * n1;
+    const int G = b.xy_budget / plane > 1 ? b.xy_budget / plane : 1;
+    const int z0 = it.tile * G;
+    const int g = (n2 - z0) < G ? (n2 - z0) : G;
+    const int cnt = g * plane;
+    const C *__restrict__ src = (b.from_in ? static_cast<const C *>(b.in) + x.in_off
+                                           : static_cast<const C *>(b.out) + x.out_off) + (long long)z0 * plane;
+    C *__restrict__ dst = static_cast<C *>(b.out) + x.out_off + (long long)z0 * plane;
+    const R sg = b.conj ? R(-1) : R(1);
+    for (int i = threadIdx.x; i < cnt; i += NTH) {
+        const C v = src[i];
+        buf[i] = cmk<C>(v.x, sg * v.y);
+    }
+    __syncthreads();
+    const C *__restrict__ tw = static_cast<const C *>(b.tw);
+    // x: g n1 rows of n0 points, element stride 1
+    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], buf, g * n1, 1, tw + b.twoff[x.size_id[0]],
+                                [=](int l) { return l * n0; });
+    // y: g n0 columns of n1 points, element stride n0
+    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], buf, g * n0, n0, tw + b.twoff[x.size_id[1]],
+                               [=](int l) { return (l / n0) * plane + (l % n0); });
+    for (int i = threadIdx.x; i < cnt; i += NTH) {
+        const C v = buf[i];
+        dst[i] = cmk<C>(v.x, sg * v.y);
+    }
+    __syncthreads();
+}
+
+template <typename R, int CLS>
+__global__ void __launch_bounds__(kXySmallThreads) fft_batched_xy(batch_args b) {
+    using C = typename cvec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    for (int i = blockIdx.x; i < b.nitems; i += gridDim.x)
+        xy_item<R, CLS, kXySmallThreads>(b, b.items[i], reinterpret_cast<C *>(smem_raw));
+}
+
+template <typename R>
+__global__ void __launch_bounds__(kXyLargeThreads) fft_batched_xy_large(batch_args b) {
+    using C = typename cvec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    for (int i = blockIdx.x; i < b.nitems; i += gridDim.x)
+        xy_item<R, 1, kXyLargeThreads>(b, b.items[i], reinterpret_cast<C *>(smem_raw));
+}
+
+template <int... CS>
+static const batched_fn *batched_xy_builds(bool dbl, std::integer_sequence<int, CS...>) {
+    static const batched_fn f[2][sizeof...(CS)] = {{&fft_batched_xy<float, CS>...}, {&fft_batched_xy<double, CS>...}};
+    return f[dbl ? 1 : 0];
+}
+
+hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
+                             hipStream_t s) {
+    static const bool configured = [] {
+        for (int d = 0; d < 2; d++) {
+            for (int c = 0; c < kBatchClasses; c++)
+                (void)hipFuncSetAttribute(
+                    reinterpret_cast<const void *>(batched_xy_builds(d != 0, std::make_integer_sequence<int, kBatchClasses>())[c]),
+                    hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        }
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched_xy_large<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched_xy_large<double>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+        return true;
+    }();
+    (void)configured;
+    if (cls < 0 || cls >= kBatchClasses || (large && cls > 1)) return hipErrorInvalidValue;
+    if (lds_bytes > (size_t)kGenericMaxLds) return hipErrorInvalidValue;
+    if (large) {
+        if (dbl) fft_batched_xy_large<double><<<dim3((unsigned)nblocks), dim3(kXyLargeThreads), lds_bytes, s>>>(b);
+        else fft_batched_xy_large<float><<<dim3((unsigned)nblocks), dim3(kXyLargeThreads), lds_bytes, s>>>(b);
+    } else {
+        const batched_fn fn = batched_xy_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
+        fn<<<dim3((unsigned)nblocks), dim3(kXySmallThreads), lds_bytes, s>>>(b);
+    }
     return hipGetLastError();
 }
 

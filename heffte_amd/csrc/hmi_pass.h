@@ -105,6 +105,8 @@ struct batch_args {
     int from_in;    // 1: read the input arrays (first phase), 0: read the output arrays in place
     int conj;       // backward transform
     int scaling;    // 0 none, 1 full (1/N of each transform), 2 symmetric; applied by the last phase
+    int xy_budget;  // fused x-y phase: elements of the planes one work item holds in LDS (a work
+                    // item of a transform takes max(1, xy_budget / (n0 n1)) consecutive z-planes)
 };
 
 }  // namespace hmi

@@ -81,6 +81,8 @@ def _load():
     lib.heffte_mi355_batch_set_stream.argtypes = [plan_p, vp]
     lib.heffte_mi355_batch_set_graph.argtypes = [plan_p, ctypes.c_int]
     lib.heffte_mi355_batch_set_autotune.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_batch_set_fused.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_batch_fused_count.argtypes = [plan_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output", "heffte_mi355_batch_graph_captures"):
         getattr(lib, f).argtypes = [plan_p]
         getattr(lib, f).restype = ctypes.c_longlong
@@ -111,7 +113,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_size_convolve_workspace", "heffte_mi355_convolve_c2c", "heffte_mi355_convolve_z2z",
     "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z", "heffte_mi355_plan_prepare",
     "heffte_mi355_batch_graph_captures", "heffte_mi355_execute_c2c_noalloc", "heffte_mi355_plan_set_timing",
-    "heffte_mi355_plan_timing_report",
+    "heffte_mi355_plan_timing_report", "heffte_mi355_batch_set_fused", "heffte_mi355_batch_fused_count",
 )
 
 
@@ -491,6 +493,17 @@ class batch_plan:
     def set_autotune(self, enable):
         """Time the tile candidates of each side length on the first call (default) or not."""
         _check(lib.heffte_mi355_batch_set_autotune(self._h, 1 if enable else 0), "set_autotune")
+
+    def set_fused(self, enable):
+        """Run the x and y axes of eligible transforms in one fused pass (default) or not."""
+        _check(lib.heffte_mi355_batch_set_fused(self._h, 1 if enable else 0), "set_fused")
+
+    def fused_count(self, double_precision=False):
+        """Transforms the fused x-y phase takes (-1 before the first call of that precision)."""
+        c = ctypes.c_int()
+        _check(lib.heffte_mi355_batch_fused_count(self._h, 1 if double_precision else 0, ctypes.byref(c)),
+               "fused_count")
+        return c.value
 
     def _run(self, direction, inarray, outarray, scaling):
         t = _dtype_name(inarray)

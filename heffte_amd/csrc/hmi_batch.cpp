@@ -92,6 +92,9 @@ batch_plan::~batch_plan() {
         (void)hipFree(d.xf);
         for (auto &ax : d.items)
             for (auto *p : ax) (void)hipFree(p);
+        for (auto &ax : d.rest)
+            for (auto *p : ax) (void)hipFree(p);
+        for (auto *p : d.xy) (void)hipFree(p);
         (void)hipFree(d.plans);
         (void)hipFree(d.twoff);
         (void)hipFree(d.kid);
@@ -139,22 +142,64 @@ void batch_plan::prepare(bool dbl) {
         plans.push_back(g);
         kids.push_back(kid);
     }
-    // work lists: every tile of every transform, one list per axis and register class
+    // Fused x-y phase: a transform whose x and y lengths both run compile-time register tiles
+    // takes the small build of the larger register class of the two when one plane fits its
+    // budget, else the 1024-thread build when a plane fits that one; work item = as many
+    // consecutive z-planes as the budget holds (at least one).
+    const int pr = dbl ? 1 : 0;
+    std::vector<int> xy_build(xf_.size(), -1);
+    {
+        std::vector<batch_item> xy[kBatchClasses + 1];
+        long long held[kBatchClasses + 1] = {};
+        for (int i = 0; i < (int)xf_.size(); i++) {
+            const batch_xform &x = xf_[i];
+            const int k0 = kids[x.size_id[0]], k1 = kids[x.size_id[1]];
+            if (!batch_xy_supported(k0) || !batch_xy_supported(k1)) continue;
+            const int cls = std::max(batch_kernel_class(k0), batch_kernel_class(k1));
+            const long long plane = (long long)xy_row_stride(x.n[0]) * x.n[1];  // LDS elements
+            int b = -1;
+            if (cls <= kXyMaxClass[pr] && plane <= kXySmallBudget[pr]) b = cls;
+            else if (cls <= kXyLargeMaxClass[pr] && plane <= kXyLargeBudget[pr]) b = kBatchClasses;
+            if (b < 0) continue;
+            xy_build[i] = b;
+            const long long budget = b == kBatchClasses ? kXyLargeBudget[pr] : kXySmallBudget[pr];
+            const long long g = std::max(1LL, budget / plane);
+            for (long long t = 0; t * g < x.n[2]; t++) xy[b].push_back({i, (int)t});
+            held[b] = std::max(held[b], std::min<long long>(g, x.n[2]) * plane);
+        }
+        d.nfused = 0;
+        for (int b : xy_build) d.nfused += b >= 0 ? 1 : 0;
+        for (int b = 0; b <= kBatchClasses; b++) {
+            if (xy[b].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
+            d.nxy[b] = (int)xy[b].size();
+            d.xy[b] = xy[b].empty() ? nullptr : upload(xy[b]);
+            d.xy_budget[b] = b == kBatchClasses ? kXyLargeBudget[pr] : kXySmallBudget[pr];
+            d.xy_lds[b] = (size_t)held[b] * elem;
+        }
+    }
+    // work lists: every tile of every transform, one list per axis and register class; for the
+    // x and y axes also the lists without the transforms of the fused phase
     for (int ax = 0; ax < 3; ax++) {
-        std::vector<batch_item> items[kBatchClasses];
+        std::vector<batch_item> items[kBatchClasses], rest[kBatchClasses];
         for (int i = 0; i < (int)xf_.size(); i++) {
             const batch_xform &x = xf_[i];
             const int col = (ax == 0) ? 1 : 0, b1 = 3 - ax - col;
             const long long nlines = (long long)x.n[col] * x.n[b1];
             const int lines = plans[x.size_id[ax]].lines;
             const long long ntiles = (nlines + lines - 1) / lines;
-            auto &list = items[batch_kernel_class(kids[x.size_id[ax]])];
-            for (long long t = 0; t < ntiles; t++) list.push_back({i, (int)t});
+            const int c = batch_kernel_class(kids[x.size_id[ax]]);
+            for (long long t = 0; t < ntiles; t++) items[c].push_back({i, (int)t});
+            if (ax < 2 && xy_build[i] < 0)
+                for (long long t = 0; t < ntiles; t++) rest[c].push_back({i, (int)t});
         }
         for (int c = 0; c < kBatchClasses; c++) {
             if (items[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
             d.nitems[ax][c] = (int)items[c].size();
             d.items[ax][c] = items[c].empty() ? nullptr : upload(items[c]);
+            if (ax < 2) {
+                d.nrest[ax][c] = (int)rest[c].size();
+                d.rest[ax][c] = rest[c].empty() ? nullptr : upload(rest[c]);
+            }
         }
     }
     d.plans = upload(plans);
@@ -169,28 +214,47 @@ void batch_plan::prepare(bool dbl) {
 
 void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int scaling, hipStream_t s) {
     device_state &d = dev_[dbl ? 1 : 0];
+    batch_args b{};
+    b.in = in;
+    b.out = out;
+    b.tw = d.tw;
+    b.xforms = d.xf;
+    b.plans = d.plans;
+    b.twoff = d.twoff;
+    b.kid = d.kid;
+    b.conj = dir > 0 ? 1 : 0;
+    // at most 2^32 - 1 work-items in a grid: beyond that the grid stride covers the rest
+    const int max_blocks = (int)(0xffffffffLL / kXyLargeThreads);
+    const bool fused = fused_ && d.nfused > 0;
+    if (fused) {
+        // fused x-y phase: reads the input arrays, writes the output arrays
+        b.axis = 0;
+        b.from_in = 1;
+        b.scaling = 0;
+        for (int k = 0; k <= kBatchClasses; k++) {
+            b.items = d.xy[k];
+            b.nitems = d.nxy[k];
+            b.xy_budget = d.xy_budget[k];
+            const int nblocks = std::min(b.nitems, max_blocks);
+            if (nblocks > 0)
+                check_hip(launch_batched_xy(dbl, k == kBatchClasses, k == kBatchClasses ? kXyLargeMaxClass[0] : k, b,
+                                            nblocks, d.xy_lds[k], s),
+                          "batched fused x-y phase");
+        }
+    }
     for (int ax = 0; ax < 3; ax++) {
-        batch_args b{};
-        b.in = in;
-        b.out = out;
-        b.tw = d.tw;
-        b.xforms = d.xf;
-        b.plans = d.plans;
-        b.twoff = d.twoff;
-        b.kid = d.kid;
         b.axis = ax;
         b.from_in = (ax == 0) ? 1 : 0;
-        b.conj = dir > 0 ? 1 : 0;
         b.scaling = (ax == 2) ? scaling : 0;
+        b.xy_budget = 0;
         for (int c = 0; c < kBatchClasses; c++) {
-            b.items = d.items[ax][c];
-            b.nitems = d.nitems[ax][c];
+            const bool rest = fused && ax < 2;
+            b.items = rest ? d.rest[ax][c] : d.items[ax][c];
+            b.nitems = rest ? d.nrest[ax][c] : d.nitems[ax][c];
             // one work item per block: the hardware schedules the blocks as others retire (a
             // persistent grid of 2048 blocks walking the list was 13 % slower on the
             // 1000-transform batch); the kernel walks the list with a grid stride, so any grid
             // size is valid
-            // at most 2^32 - 1 work-items in a grid: beyond that the grid stride covers the rest
-            const int max_blocks = (int)(0xffffffffLL / (dbl ? 128 : 256));
             const int nblocks = std::min(b.nitems, max_blocks);
             if (nblocks > 0) check_hip(launch_batched(dbl, c, b, nblocks, d.lds[c], s), "batched FFT phase");
         }
@@ -221,7 +285,7 @@ void batch_plan::execute(int dir, bool dbl, const void *in, void *out, int scali
         return;
     }
     const int g = dbl ? 1 : 0;
-    graph_key key{dir, scaling, in, const_cast<void *>(out), stream_};
+    graph_key key{dir, scaling, in, const_cast<void *>(out), stream_, fused_};
     auto found = graphs_[g].find(key);
     if (found == graphs_[g].end()) {
         // capture the three phase launches once per (direction, scale, arrays, stream) and replay

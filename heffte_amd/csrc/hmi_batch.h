@@ -1,6 +1,7 @@
-// Batch of independent c2c 3D transforms of different sizes, run as three
-// phases (one per axis; one launch per register class of tile) over device work lists, optionally
-// replayed from a captured hipGraph.  There is no counterpart in heFFTe, whose
+// Batch of independent c2c 3D transforms of different sizes, run as phases over device work
+// lists (one launch per register class of tile), optionally replayed from a captured hipGraph:
+// the fused x-y phase for the transforms whose planes and tiles allow it, the x and y phases for
+// the others, then the z phase for all.  There is no counterpart in heFFTe, whose
 // batches share one geometry (include/heffte_fft3d.h:392-402).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -48,6 +49,11 @@ class batch_plan {
     void set_graph(bool on) { use_graph_ = on; }
     // off: every length takes batch_tuner::fallback (no timing runs on the first execute)
     void set_autotune(bool on) { autotune_ = on; }
+    // off: every transform takes the separate x and y phases (the fused x-y phase is on by default)
+    void set_fused(bool on) { fused_ = on; }
+    // transforms the fused x-y phase takes (the work lists are built on the first execute of each
+    // precision; -1 before that)
+    int fused_count(bool dbl) const { return dev_[dbl ? 1 : 0].ready ? dev_[dbl ? 1 : 0].nfused : -1; }
     int count() const { return (int)xf_.size(); }
     long long input_elems() const { return in_extent_; }
     long long output_elems() const { return out_extent_; }
@@ -66,6 +72,16 @@ class batch_plan {
         // own build of the batched kernel
         batch_item *items[3][kBatchClasses] = {};
         int nitems[3][kBatchClasses] = {};
+        // the x and y work lists restricted to the transforms the fused phase does not take
+        batch_item *rest[2][kBatchClasses] = {};
+        int nrest[2][kBatchClasses] = {};
+        // fused x-y work lists: one per small build (register class), the last for the
+        // 1024-thread build; planes per work item set by xy_budget, LDS bytes per block
+        batch_item *xy[kBatchClasses + 1] = {};
+        int nxy[kBatchClasses + 1] = {};
+        int xy_budget[kBatchClasses + 1] = {};
+        size_t xy_lds[kBatchClasses + 1] = {};
+        int nfused = 0;
         generic_plan *plans = nullptr;
         int *twoff = nullptr;
         int *kid = nullptr;
@@ -77,9 +93,11 @@ class batch_plan {
         const void *in;
         void *out;
         hipStream_t stream;
+        bool fused;
         bool operator<(const graph_key &o) const {
             if (dir != o.dir) return dir < o.dir;
             if (scaling != o.scaling) return scaling < o.scaling;
+            if (fused != o.fused) return fused < o.fused;
             if (in != o.in) return std::less<const void *>()(in, o.in);
             if (out != o.out) return std::less<void *>()(out, o.out);
             return std::less<hipStream_t>()(stream, o.stream);
@@ -99,6 +117,7 @@ class batch_plan {
     hipStream_t stream_ = nullptr;
     bool use_graph_ = true;
     bool autotune_ = true;
+    bool fused_ = true;
     std::map<int, batch_choice> forced_;  // probe plans of the tuner
     friend class batch_tuner;
     device_state dev_[2];

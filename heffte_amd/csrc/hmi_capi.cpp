@@ -318,6 +318,19 @@ int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable) {
     return HMI_SUCCESS;
 }
 
+int heffte_mi355_batch_set_fused(heffte_mi355_batch batch, int enable) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    batch->impl->set_fused(enable != 0);
+    return HMI_SUCCESS;
+}
+
+int heffte_mi355_batch_fused_count(heffte_mi355_batch batch, int double_precision, int *count) {
+    if (!valid_batch(batch)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed batch");
+    if (!count) return fail(HMI_ERROR_INVALID_ARGUMENT, "null count pointer");
+    *count = batch->impl->fused_count(double_precision != 0);
+    return HMI_SUCCESS;
+}
+
 long long heffte_mi355_batch_size_input(heffte_mi355_batch batch) {
     return valid_batch(batch) ? batch->impl->input_elems() : -1;
 }

@@ -57,15 +57,22 @@ const char *kernels_build_stamp();
 hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
 
 // Fused x-y phase of a batch: each work item loads whole x-y planes (contiguous) of one
-// transform into LDS, runs the x-axis and y-axis FFTs there and stores the planes back, so the x
-// and y axes cost one read and one write of the array instead of two each.  Transforms whose x
-// and y lengths both have register tiles (batch_xy_supported) take it.  Two builds: 512 threads
-// with planes of at most kXySmallBudget elements (one per register class of tile, cls = the
-// larger class of the two lengths) and 1024 threads for planes up to kXyLargeBudget elements
-// (tiles of the first two classes only).
-constexpr int kXySmallThreads = 512, kXyLargeThreads = 1024;
+// transform into LDS (n1 rows of xy_row_stride(n0) elements per plane), runs the x-axis and
+// y-axis FFTs there and stores the planes back, so the x and y axes cost one read and one write
+// of the array instead of two each.  Transforms whose x and y lengths both have register tiles
+// (batch_xy_supported) of a register class up to kXyMaxClass take it.  Builds: 256 threads with
+// planes of at most kXySmallBudget elements (one per register class, cls = the larger class of
+// the two lengths), and for complex<float> 512 threads with planes up to kXyLargeBudget elements
+// (tiles of the classes up to kXyLargeMaxClass; -1: no such build).  Every build runs without
+// VGPR spills.
+constexpr int kXySmallThreads = 256, kXyLargeThreads = 512;
 constexpr int kXySmallBudget[2] = {8192, 4096};    // elements: 64 KiB of LDS (complex<float>, <double>)
-constexpr int kXyLargeBudget[2] = {20480, 10240};  // elements: 160 KiB of LDS
+constexpr int kXyLargeBudget[2] = {20480, 0};      // elements: 160 KiB of LDS
+constexpr int kXyMaxClass[2] = {2, 0};             // register classes with a small build
+constexpr int kXyLargeMaxClass[2] = {2, -1};       // register classes the 512-thread build runs
+// Row stride (elements) of an x-y plane of rows of n0 elements in the LDS of the fused phase: odd,
+// so that 32 lanes reading one position of 32 consecutive rows hit distinct banks.
+constexpr int xy_row_stride(int n0) { return n0 | 1; }
 bool batch_xy_supported(int kid);
 hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
                              hipStream_t s);

@@ -834,22 +834,31 @@ __device__ __forceinline__ void lds_line_stages(C (&v)[E], C *lb, int es, const 
     }
 }
 
-// n-point FFTs of nlines lines held in LDS (line l at buf + base_of(l), element stride es), by
-// NTH threads in rounds of NTH / P lines.  ACROSS = false: consecutive lanes walk a line (lines
-// whose elements are adjacent); true: consecutive lanes walk consecutive lines (lines whose
-// starts are adjacent), so the LDS accesses of a wave stay on consecutive slots either way.
-template <typename RL, int E, int NTH, bool ACROSS, typename C, typename BaseF>
-__device__ __forceinline__ void lds_lines_fft(C *buf, int nlines, int es, const C *__restrict__ tw, BaseF base_of) {
+// n-point FFTs of the lines of the x-y planes held in LDS (the dynamic shared memory of the
+// kernel): row r of plane p at (p n1 + r) rs, rs = xy_row_stride(n0) (odd), planes ps = n1 rs
+// apart.  ACROSS = false: the rows (line l at l rs, element stride 1); ACROSS = true: the columns
+// (line l = p n0 + c at p ps + c, element stride rs).  Consecutive lanes take consecutive lines
+// at one position t of their lines: the odd row stride puts the lanes of a row pass on distinct
+// LDS banks, consecutive columns are consecutive addresses.  NTH threads, rounds of NTH / P
+// lines.
+template <typename RL, int E, int NTH, bool ACROSS, typename C>
+__device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps, const C *__restrict__ tw) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *buf = reinterpret_cast<C *>(smem_raw);
     constexpr int N = RL::n(), P = N / E, LPI = NTH / P;
     static_assert(LPI >= 1, "a line needs at most the block's threads");
+    const int es = ACROSS ? rs : 1;
     const int tid = threadIdx.x;
-    const int l0 = ACROSS ? tid % LPI : tid / P;
-    const int t = ACROSS ? tid / LPI : tid % P;
+    const int l0 = tid % LPI;
+    // with LPI a multiple of 64 a wave holds 64 consecutive lines at one position t: t, the
+    // twiddles and every element offset are uniform over the wave (scalar registers and loads)
+    int t = tid / LPI;
+    if constexpr (LPI % 64 == 0) t = __builtin_amdgcn_readfirstlane(t);
     const bool inb = tid < LPI * P;
     for (int first = 0; first < nlines; first += LPI) {
         const int line = first + l0;
         const bool valid = inb && line < nlines;
-        C *lb = buf + (valid ? base_of(line) : 0);
+        C *lb = buf + (!valid ? 0 : (ACROSS ? (line / n0) * ps + line % n0 : line * rs));
         C v[E];
         if (valid) {
 #pragma unroll
@@ -871,13 +880,12 @@ static const bool kXyOk[] = {HMI_BATCH_TABLE(HMI_XY_OK)};
 bool batch_xy_supported(int kid) { return kid >= 0 && kXyOk[kid]; }
 
 // the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
-template <typename R, int CLS, int NTH, bool ACROSS, typename BaseF>
-__device__ __forceinline__ void xy_axis(int kid, typename cvec<R>::type *buf, int nlines, int es,
-                                        const typename cvec<R>::type *tw, BaseF base_of) {
+template <typename R, int CLS, int NTH, bool ACROSS>
+__device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int ps, const typename cvec<R>::type *tw) {
 #define HMI_XY_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                     \
         if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1) \
-            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(buf, nlines, es, tw, base_of); \
+            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw); \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_XY_CASE)
@@ -887,85 +895,124 @@ __device__ __forceinline__ void xy_axis(int kid, typename cvec<R>::type *buf, in
 }
 
 // Fused x-y phase: work item = (transform, group of G consecutive z-planes), G = max(1, budget /
-// (n0 n1)).  The planes are contiguous in memory: one coalesced load into LDS (conjugated for the
-// backward transform), the n0-point FFTs of the G n1 rows, the n1-point FFTs of the G n0
-// columns, one coalesced store (conjugated back).  No scale: the z phase applies it.
+// (n1 rs)), rs = xy_row_stride(n0).  The planes are contiguous in memory: one coalesced load into
+// the padded rows in LDS (conjugated for the backward transform), the n0-point FFTs of the G n1
+// rows, the n1-point FFTs of the G n0 columns, one coalesced store (conjugated back).  No scale:
+// the z phase applies it.
 template <typename R, int CLS, int NTH>
 __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &it, typename cvec<R>::type *buf) {
     using C = typename cvec<R>::type;
     const batch_xform x = b.xforms[it.xform];
     const int n0 = x.n[0], n1 = x.n[1], n2 = x.n[2];
-    const int plane = n0 * n1;
-    const int G = b.xy_budget / plane > 1 ? b.xy_budget / plane : 1;
+    const int rs = xy_row_stride(n0), ps = n1 * rs;
+    const int G = b.xy_budget / ps > 1 ? b.xy_budget / ps : 1;
     const int z0 = it.tile * G;
     const int g = (n2 - z0) < G ? (n2 - z0) : G;
-    const int cnt = g * plane;
+    const int cnt = g * n1 * n0;
+    const long long zoff = (long long)z0 * n1 * n0;
     const C *__restrict__ src = (b.from_in ? static_cast<const C *>(b.in) + x.in_off
-                                           : static_cast<const C *>(b.out) + x.out_off) + (long long)z0 * plane;
-    C *__restrict__ dst = static_cast<C *>(b.out) + x.out_off + (long long)z0 * plane;
+                                           : static_cast<const C *>(b.out) + x.out_off) + zoff;
+    C *__restrict__ dst = static_cast<C *>(b.out) + x.out_off + zoff;
     const R sg = b.conj ? R(-1) : R(1);
-    for (int i = threadIdx.x; i < cnt; i += NTH) {
-        const C v = src[i];
-        buf[i] = cmk<C>(v.x, sg * v.y);
+    // element i of the group sits in LDS row i / n0 (the g n1 rows back to back), column i % n0;
+    // walking i by NTH moves the pair by (NTH / n0, NTH % n0).  Loads go out U at a time before
+    // their LDS stores, so that U loads per thread are in flight.
+    constexpr int U = 16;
+    const int dr = NTH / n0, dc = NTH % n0;
+    const int r0 = (int)threadIdx.x / n0, c0 = (int)threadIdx.x % n0;
+    {
+        int r = r0, c = c0;
+        for (int i0 = threadIdx.x; i0 < cnt; i0 += U * NTH) {
+            C v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (i0 + u * NTH < cnt) v[u] = src[i0 + u * NTH];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + u * NTH < cnt) buf[r * rs + c] = cmk<C>(v[u].x, sg * v[u].y);
+                r += dr;
+                c += dc;
+                if (c >= n0) c -= n0, r++;
+            }
+        }
     }
     __syncthreads();
     const C *__restrict__ tw = static_cast<const C *>(b.tw);
-    // x: g n1 rows of n0 points, element stride 1
-    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], buf, g * n1, 1, tw + b.twoff[x.size_id[0]],
-                                [=](int l) { return l * n0; });
-    // y: g n0 columns of n1 points, element stride n0
-    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], buf, g * n0, n0, tw + b.twoff[x.size_id[1]],
-                               [=](int l) { return (l / n0) * plane + (l % n0); });
-    for (int i = threadIdx.x; i < cnt; i += NTH) {
-        const C v = buf[i];
-        dst[i] = cmk<C>(v.x, sg * v.y);
+    // x: g n1 rows of n0 points; y: g n0 columns of n1 points
+    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], g * n1, n0, rs, ps, tw + b.twoff[x.size_id[0]]);
+    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]]);
+    {
+        int r = r0, c = c0;
+        for (int i0 = threadIdx.x; i0 < cnt; i0 += U * NTH) {
+            C v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (i0 + u * NTH < cnt) v[u] = buf[r * rs + c];
+                r += dr;
+                c += dc;
+                if (c >= n0) c -= n0, r++;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (i0 + u * NTH < cnt) dst[i0 + u * NTH] = cmk<C>(v[u].x, sg * v[u].y);
+        }
     }
     __syncthreads();
 }
 
+// Every build at 2 waves per SIMD (up to 256 VGPRs: no tile spills): two 256-thread blocks per
+// CU with 64 KiB of LDS each, or one 512-thread block with up to 160 KiB.
+constexpr int kXyWaves[kBatchClasses][2] = {{2, 2}, {2, 2}, {2, 2}};
+
 template <typename R, int CLS>
-__global__ void __launch_bounds__(kXySmallThreads) fft_batched_xy(batch_args b) {
+__global__ void __launch_bounds__(kXySmallThreads)
+__attribute__((amdgpu_waves_per_eu(kXyWaves[CLS][sizeof(R) == 8])))
+fft_batched_xy(batch_args b) {
     using C = typename cvec<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     for (int i = blockIdx.x; i < b.nitems; i += gridDim.x)
         xy_item<R, CLS, kXySmallThreads>(b, b.items[i], reinterpret_cast<C *>(smem_raw));
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kXyLargeThreads) fft_batched_xy_large(batch_args b) {
-    using C = typename cvec<R>::type;
+__global__ void __launch_bounds__(kXyLargeThreads) __attribute__((amdgpu_waves_per_eu(2)))
+fft_batched_xy_large(batch_args b) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     for (int i = blockIdx.x; i < b.nitems; i += gridDim.x)
-        xy_item<R, 1, kXyLargeThreads>(b, b.items[i], reinterpret_cast<C *>(smem_raw));
+        xy_item<float, kXyLargeMaxClass[0], kXyLargeThreads>(b, b.items[i], reinterpret_cast<float2 *>(smem_raw));
 }
+static_assert(kXyLargeMaxClass[1] < 0, "the 1024-thread build is complex<float> only");
 
+template <typename R, int CLS>
+constexpr batched_fn xy_build() {
+    if constexpr (CLS <= kXyMaxClass[sizeof(R) == 8]) return &fft_batched_xy<R, CLS>;
+    else return nullptr;
+}
 template <int... CS>
 static const batched_fn *batched_xy_builds(bool dbl, std::integer_sequence<int, CS...>) {
-    static const batched_fn f[2][sizeof...(CS)] = {{&fft_batched_xy<float, CS>...}, {&fft_batched_xy<double, CS>...}};
+    static const batched_fn f[2][sizeof...(CS)] = {{xy_build<float, CS>()...}, {xy_build<double, CS>()...}};
     return f[dbl ? 1 : 0];
 }
 
 hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
                              hipStream_t s) {
     static const bool configured = [] {
-        for (int d = 0; d < 2; d++) {
-            for (int c = 0; c < kBatchClasses; c++)
-                (void)hipFuncSetAttribute(
-                    reinterpret_cast<const void *>(batched_xy_builds(d != 0, std::make_integer_sequence<int, kBatchClasses>())[c]),
-                    hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
-        }
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched_xy_large<float>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
-        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched_xy_large<double>),
+        for (int d = 0; d < 2; d++)
+            for (int c = 0; c < kBatchClasses; c++) {
+                const batched_fn fn = batched_xy_builds(d != 0, std::make_integer_sequence<int, kBatchClasses>())[c];
+                if (fn)
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+            }
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&fft_batched_xy_large),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
         return true;
     }();
     (void)configured;
-    if (cls < 0 || cls >= kBatchClasses || (large && cls > 1)) return hipErrorInvalidValue;
+    if (cls < 0 || cls >= kBatchClasses || cls > (large ? kXyLargeMaxClass : kXyMaxClass)[dbl ? 1 : 0])
+        return hipErrorInvalidValue;
     if (lds_bytes > (size_t)kGenericMaxLds) return hipErrorInvalidValue;
     if (large) {
-        if (dbl) fft_batched_xy_large<double><<<dim3((unsigned)nblocks), dim3(kXyLargeThreads), lds_bytes, s>>>(b);
-        else fft_batched_xy_large<float><<<dim3((unsigned)nblocks), dim3(kXyLargeThreads), lds_bytes, s>>>(b);
+        fft_batched_xy_large<<<dim3((unsigned)nblocks), dim3(kXyLargeThreads), lds_bytes, s>>>(b);
     } else {
         const batched_fn fn = batched_xy_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
         fn<<<dim3((unsigned)nblocks), dim3(kXySmallThreads), lds_bytes, s>>>(b);

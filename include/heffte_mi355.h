@@ -223,7 +223,7 @@ int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const
  * (one per axis; each one launch per register class of tile) over device work lists, replayed
  * from a captured hipGraph unless
  * heffte_mi355_batch_set_graph(batch, 0) was called; one instantiated graph is kept per
- * (direction, scale, input, output, stream), so calls that alternate between a few argument
+ * (direction, scale, input, output, stream, fused setting), so calls that alternate between a few argument
  * sets replay without capturing again.  scale applies per transform (1/N_i).
  * Aliasing: the output ranges of the transforms must be disjoint (checked at creation);
  * input == output only when in_offsets[i] == out_offsets[i] for every i, otherwise the input and
@@ -243,6 +243,16 @@ int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
  * ignored and rewritten).  enable = 0 takes the untimed defaults.  The
  * first call then runs timing launches on a stream of its own: do it outside stream capture. */
 int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable);
+/* Fused x-y phase (on by default): a transform whose x and y lengths both have compile-time
+ * register tiles and whose x-y plane fits the LDS of one block (8192 complex<float> / 4096
+ * complex<double> elements, or 20480 complex<float> elements in the 1024-thread build) runs its
+ * x and y axes in one pass over its planes (one read and one write of the array instead of two);
+ * the others take the separate x and y phases.  enable = 0 sends every transform through the
+ * separate phases.  heffte_mi355_batch_fused_count stores in *count how many transforms the
+ * fused phase takes in the given precision (-1 before the first forward/backward call of that
+ * precision). */
+int heffte_mi355_batch_set_fused(heffte_mi355_batch batch, int enable);
+int heffte_mi355_batch_fused_count(heffte_mi355_batch batch, int double_precision, int *count);
 /* Elements (complex) the input / output arrays must hold. */
 long long heffte_mi355_batch_size_input(heffte_mi355_batch batch);
 long long heffte_mi355_batch_size_output(heffte_mi355_batch batch);

@@ -203,3 +203,63 @@ def test_autotune_cache_from_another_build_is_ignored(hmi, cuda, oracle, tmp_pat
     assert lines[0].startswith("# heffte_mi355 batch tuning, build ") and "2000" not in lines[0]
     assert "20 0 0 3 2,2,5" not in lines[1:]
     assert any(line.split()[:2] == ["20", "0"] for line in lines[1:])
+
+
+# elements per thread of the compile-time tile of each side length (0: the staged tile of
+# length 105, which the fused x-y phase does not run)
+TILE_E = {16: 16, 20: 10, 24: 12, 27: 9, 28: 14, 32: 8, 40: 10, 45: 15, 48: 12, 56: 14, 63: 21, 64: 8,
+          80: 10, 96: 12, 112: 14, 128: 16, 105: 0}
+
+
+def fused_eligible(size, dtype):
+    """Whether the fused x-y phase takes a transform: both x and y lengths have register tiles;
+    register class = the larger of the two (E <= 12: 0, E <= 16: 1, else 2); complex<float>: a
+    256-thread build for every class and planes of at most 8192 elements, a 512-thread build for
+    planes up to 20480; complex<double>: class 0, planes up to 4096 (a plane counts n1 rows of
+    n0 | 1 elements)."""
+    e0, e1 = TILE_E.get(size[0], 0), TILE_E.get(size[1], 0)
+    if e0 == 0 or e1 == 0:
+        return False
+    cls = max(0 if e <= 12 else (1 if e <= 16 else 2) for e in (e0, e1))
+    plane = (size[0] | 1) * size[1]  # LDS rows padded to an odd length
+    if dtype == "complex64":
+        return plane <= 20480
+    return cls == 0 and plane <= 4096
+
+
+# every build of the fused phase, z-plane groups with a remainder, columns with 3 threads per line
+# (27, 45, 63 on y), transforms left to the separate phases (105, length 1, planes beyond the
+# budget, the 21-element tile in complex<double>)
+FUSED_SIZES = [(16, 16, 33), (27, 45, 5), (63, 64, 7), (20, 63, 4), (128, 128, 3), (96, 112, 2), (128, 80, 2),
+               (105, 16, 4), (16, 105, 4), (63, 128, 3), (40, 24, 48), (112, 63, 2), (1, 32, 16), (64, 1, 8),
+               (45, 27, 9), (56, 28, 6), (32, 128, 5)]
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_fused_xy_phase(hmi, cuda, oracle, dtype):
+    """Fused x-y phase against the oracle transform by transform (forward), the round trip
+    (backward, full scale, in place), and the separate-phase path of the same plan; the number of
+    transforms it takes follows the eligibility rule above."""
+    import torch
+    sizes = FUSED_SIZES
+    plan = hmi.batch_plan(sizes)
+    plan.set_autotune(False)
+    dbl = dtype == "complex128"
+    assert plan.fused_count(dbl) == -1
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in sizes])[:-1]
+    x = fc.uniform_complex(plan.size_input(), 31)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    tx = torch.from_numpy(x).to(cuda).to(tdt)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty)
+    assert plan.fused_count(dbl) == sum(fused_eligible(s, dtype) for s in sizes)
+    y = fc.to_host(ty)
+    for s, off, r in zip(sizes, offsets, oracle_each(oracle, sizes, offsets, x, -1, 0)):
+        assert fc.rel_l2(y[off:off + r.size], r) <= TOL[dtype], s
+    plan.set_fused(False)
+    ts = torch.empty_like(tx)
+    plan.forward(tx, ts)
+    assert fc.rel_l2(fc.to_host(ts), y) <= TOL[dtype]
+    plan.set_fused(True)
+    plan.backward(ty, ty, hmi.scale.full)
+    assert fc.rel_l2(fc.to_host(ty), x) <= TOL[dtype]

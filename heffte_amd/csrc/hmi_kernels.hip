@@ -639,22 +639,25 @@ __global__ void __launch_bounds__(256) fft_generic(pass_args a, generic_plan g) 
 // radices).  P = n / E threads per line; P * T <= 256 (complex<float>, 256-thread blocks) and
 // P * (T / 2) <= 128 (complex<double>, 128-thread blocks); LDS = (T + 1) rows stays <= 40 KiB.
 // E = 0 marks a staged tile (staged_tile: one butterfly per work unit and stage, 2 T rows of LDS).
+// The pad shift of each tile minimises the LDS bank conflicts of its stage exchanges, in both the
+// contiguous and the column-chunk mapping, under the gfx950 rule (ds_read_b64: 32 lanes per pass
+// over 64 banks; ds_write_b64: 16 lanes over 32 banks), at no more LDS than one pad per 16.
 #define HMI_BATCH_TABLE(X)            \
     X(0, 16, 16, 256, 4, 16)          \
-    X(1, 20, 10, 128, 4, 2, 2, 5)     \
-    X(2, 24, 12, 128, 4, 3, 4, 2)     \
+    X(1, 20, 10, 128, 5, 2, 2, 5)     \
+    X(2, 24, 12, 128, 5, 3, 4, 2)     \
     X(3, 27, 9, 84, 4, 3, 3, 3)       \
-    X(4, 28, 14, 128, 4, 2, 2, 7)     \
-    X(5, 32, 8, 64, 3, 4, 8)          \
-    X(6, 40, 10, 64, 4, 2, 2, 2, 5)   \
-    X(7, 45, 15, 84, 4, 3, 3, 5)      \
-    X(8, 48, 12, 64, 4, 4, 4, 3)      \
-    X(9, 56, 14, 64, 4, 2, 2, 2, 7)   \
+    X(4, 28, 14, 128, 5, 2, 2, 7)     \
+    X(5, 32, 8, 64, 4, 4, 8)          \
+    X(6, 40, 10, 64, 6, 2, 2, 2, 5)   \
+    X(7, 45, 15, 84, 5, 3, 3, 5)      \
+    X(8, 48, 12, 64, 3, 4, 4, 3)      \
+    X(9, 56, 14, 64, 6, 2, 2, 2, 7)   \
     X(10, 63, 21, 64, 4, 3, 3, 7)     \
     X(11, 64, 8, 32, 3, 8, 8)         \
-    X(12, 80, 10, 32, 4, 2, 2, 2, 2, 5) \
+    X(12, 80, 10, 32, 7, 2, 2, 2, 2, 5) \
     X(13, 96, 12, 32, 4, 4, 4, 2, 3)  \
-    X(14, 112, 14, 32, 4, 2, 2, 2, 2, 7) \
+    X(14, 112, 14, 32, 7, 2, 2, 2, 2, 7) \
     X(15, 128, 16, 32, 4, 16, 8)       \
     X(16, 105, 0, 16, 0, 3, 5, 7)
 

@@ -291,7 +291,9 @@ def test_dct_256_double_round_trip(hmi, cuda):
 
 def test_batch_1000_mixed_round_trip(hmi, cuda):
     """The 1,000-transform batch (seed 4 sides, U[-1,1] values): batched forward and backward
-    (scale::full) returns the input, and each transform's forward equals the plain fft3d plan."""
+    (scale::full) returns the input; every transform's forward equals the plain fft3d plan's
+    complex<double> forward of the same input (a separate kernel path: the fft3d passes), to the
+    complex<float> bound; the fused x-y phase (899 transforms) and the separate phases agree."""
     import torch
     rng = np.random.default_rng(4)
     sizes = [tuple(int(v) for v in rng.choice(SMOOTH, 3)) for _ in range(1000)]
@@ -302,14 +304,26 @@ def test_batch_1000_mixed_round_trip(hmi, cuda):
     x = (torch.rand(sum(counts), dtype=torch.complex64, device=cuda, generator=g) * 2 - (1 + 1j)).contiguous()
     y = torch.empty_like(x)
     plan.forward(x, y)
+    assert plan.fused_count(False) == 899
     offs = np.cumsum([0] + counts)[:-1]
-    for idx in (0, 1, 499, 998, 999):
+    singles = {}
+    worst = 0.0
+    for idx in range(len(sizes)):
         s, o, c = sizes[idx], int(offs[idx]), counts[idx]
-        b = hmi.box3d([0, 0, 0], [v - 1 for v in s])
-        single = hmi.fft3d(hmi.backend.mi355, b, b)
+        if s not in singles:
+            b = hmi.box3d([0, 0, 0], [v - 1 for v in s])
+            singles[s] = hmi.fft3d(hmi.backend.mi355, b, b)
         ref = torch.empty(c, dtype=torch.complex128, device=cuda)
-        single.forward(x[o:o + c].to(torch.complex128), ref)
-        assert rel(y[o:o + c], ref) <= 5e-6, (idx, s)
+        singles[s].forward(x[o:o + c].to(torch.complex128), ref)
+        e = rel(y[o:o + c], ref)
+        assert e <= 5e-6, (idx, s, e)
+        worst = max(worst, e)
+    print(f"worst per-transform rel L2 vs fft3d complex<double>: {worst:.2e}")
+    plan.set_fused(False)
+    ys = torch.empty_like(x)
+    plan.forward(x, ys)
+    assert rel(ys, y) <= 5e-6
+    plan.set_fused(True)
     z = torch.empty_like(x)
     plan.backward(y, z, hmi.scale.full)
     assert rel(z, x) <= 5e-6

@@ -843,9 +843,12 @@ __device__ __forceinline__ void lds_line_stages(C (&v)[E], C *lb, int es, const 
 // (line l = p n0 + c at p ps + c, element stride rs).  Consecutive lanes take consecutive lines
 // at one position t of their lines: the odd row stride puts the lanes of a row pass on distinct
 // LDS banks, consecutive columns are consecutive addresses.  NTH threads, rounds of NTH / P
-// lines.
-template <typename RL, int E, int NTH, bool ACROSS, typename C>
-__device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps, const C *__restrict__ tw) {
+// lines.  The columns' results go from registers straight to global memory: dst holds the
+// planes as n1 rows of n0 elements (consecutive lanes store consecutive columns), each value
+// times (1, sg) on the way.
+template <typename RL, int E, int NTH, bool ACROSS, typename C, typename R>
+__device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps, const C *__restrict__ tw,
+                                              C *__restrict__ dst, int n1, R sg) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *buf = reinterpret_cast<C *>(smem_raw);
     constexpr int N = RL::n(), P = N / E, LPI = NTH / P;
@@ -870,8 +873,14 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
         __syncthreads();
         lds_line_stages<0, RL, E, P>(v, lb, es, tw, t, valid);
         if (valid) {
+            if constexpr (ACROSS) {
+                C *gb = dst + (long long)(line / n0) * n0 * n1 + line % n0;
 #pragma unroll
-            for (int e = 0; e < E; e++) lb[(t + P * e) * es] = v[e];
+                for (int e = 0; e < E; e++) gb[(t + P * e) * n0] = cmk<C>(v[e].x, sg * v[e].y);
+            } else {
+#pragma unroll
+                for (int e = 0; e < E; e++) lb[(t + P * e) * es] = v[e];
+            }
         }
         __syncthreads();
     }
@@ -884,11 +893,12 @@ bool batch_xy_supported(int kid) { return kid >= 0 && kXyOk[kid]; }
 
 // the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
 template <typename R, int CLS, int NTH, bool ACROSS>
-__device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int ps, const typename cvec<R>::type *tw) {
+__device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int ps, const typename cvec<R>::type *tw,
+                                        typename cvec<R>::type *dst, int n1, R sg) {
 #define HMI_XY_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                     \
         if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1) \
-            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw); \
+            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw, dst, n1, sg); \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_XY_CASE)
@@ -900,8 +910,9 @@ __device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int
 // Fused x-y phase: work item = (transform, group of G consecutive z-planes), G = max(1, budget /
 // (n1 rs)), rs = xy_row_stride(n0).  The planes are contiguous in memory: one coalesced load into
 // the padded rows in LDS (conjugated for the backward transform), the n0-point FFTs of the G n1
-// rows, the n1-point FFTs of the G n0 columns, one coalesced store (conjugated back).  No scale:
-// the z phase applies it.
+// rows, the n1-point FFTs of the G n0 columns, whose results go from registers to global memory
+// (conjugated back; consecutive lanes write consecutive columns).  No scale: the z phase applies
+// it.
 template <typename R, int CLS, int NTH>
 __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &it, typename cvec<R>::type *buf) {
     using C = typename cvec<R>::type;
@@ -941,26 +952,9 @@ __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &i
     }
     __syncthreads();
     const C *__restrict__ tw = static_cast<const C *>(b.tw);
-    // x: g n1 rows of n0 points; y: g n0 columns of n1 points
-    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], g * n1, n0, rs, ps, tw + b.twoff[x.size_id[0]]);
-    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]]);
-    {
-        int r = r0, c = c0;
-        for (int i0 = threadIdx.x; i0 < cnt; i0 += U * NTH) {
-            C v[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                if (i0 + u * NTH < cnt) v[u] = buf[r * rs + c];
-                r += dr;
-                c += dc;
-                if (c >= n0) c -= n0, r++;
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++)
-                if (i0 + u * NTH < cnt) dst[i0 + u * NTH] = cmk<C>(v[u].x, sg * v[u].y);
-        }
-    }
-    __syncthreads();
+    // x: g n1 rows of n0 points; y: g n0 columns of n1 points, stored from registers
+    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], g * n1, n0, rs, ps, tw + b.twoff[x.size_id[0]], dst, n1, sg);
+    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]], dst, n1, sg);
 }
 
 // Every build at 2 waves per SIMD (up to 256 VGPRs: no tile spills): two 256-thread blocks per

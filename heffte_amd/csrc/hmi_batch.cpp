@@ -157,8 +157,9 @@ void batch_plan::prepare(bool dbl) {
             if (!batch_xy_supported(k0) || !batch_xy_supported(k1)) continue;
             const int cls = std::max(batch_kernel_class(k0), batch_kernel_class(k1));
             const long long plane = (long long)xy_row_stride(x.n[0]) * x.n[1];  // LDS elements
+            const bool staged = batch_tile_staged(k0) || batch_tile_staged(k1);
             int b = -1;
-            if (cls <= kXyMaxClass[pr] && plane <= kXySmallBudget[pr]) b = cls;
+            if (cls <= kXyMaxClass[pr] && plane <= kXySmallBudget[pr] && !(staged && cls > 0)) b = cls;
             else if (cls <= kXyLargeMaxClass[pr] && plane <= kXyLargeBudget[pr]) b = kBatchClasses;
             if (b < 0) continue;
             xy_build[i] = b;

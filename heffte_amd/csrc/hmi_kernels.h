@@ -74,6 +74,9 @@ constexpr int kXyLargeMaxClass[2] = {2, -1};       // register classes the 512-t
 // so that 32 lanes reading one position of 32 consecutive rows hit distinct banks.
 constexpr int xy_row_stride(int n0) { return n0 | 1; }
 bool batch_xy_supported(int kid);
+// the tile of kid is a staged tile (no register form): the 256-thread builds carry it in class 0
+// only (a transform with a staged side and a side of a higher class takes the 512-thread build)
+bool batch_tile_staged(int kid);
 hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
                              hipStream_t s);
 

@@ -886,10 +886,88 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
     }
 }
 
-// Register tiles the fused phase can run: compile-time tiles with E > 0 (not the staged tile)
-#define HMI_XY_OK(K, N, E, T, PS, ...) (E > 0),
+// The stages of one line as lds_line_stages, for lengths whose radices do not all divide one
+// elements-per-thread count (105 = 3 5 7): each stage reads its butterflies' inputs from the
+// line's LDS slots (thread t takes butterflies t, t + P, ... < n / R), a barrier, then writes
+// their outputs; the last stage of a column (ACROSS) writes to global memory instead (gb: the
+// column's first element, rows n0 apart, values times (1, sg)).
+template <int S, typename RL, int P, bool ACROSS, typename C, typename R>
+__device__ __forceinline__ void staged_line_stages(C *lb, int es, const C *__restrict__ tw, int t, bool valid, C *gb,
+                                                   int n0, R sg) {
+    constexpr int Rr = RL::at(S), Ls = RL::ls(S), N = RL::n(), NBT = N / Rr, NBS = (NBT + P - 1) / P;
+    constexpr bool last = (S + 1 == RL::count);
+    constexpr int TWO = RL::twoff(S);
+    C x[NBS][Rr];
+#pragma unroll
+    for (int j = 0; j < NBS; j++) {
+        const int b = t + P * j;
+        if (valid && b < NBT) {
+#pragma unroll
+            for (int p = 0; p < Rr; p++) x[j][p] = lb[(b + NBT * p) * es];
+            if constexpr (Ls > 1) {
+                const C *w = tw + TWO + (b % Ls) * (Rr - 1);
+#pragma unroll
+                for (int p = 1; p < Rr; p++) x[j][p] = cmul(x[j][p], w[p - 1]);
+            }
+            dft<Rr>::run(x[j]);
+        }
+    }
+    __syncthreads();  // every read of the stage before its writes
+#pragma unroll
+    for (int j = 0; j < NBS; j++) {
+        const int b = t + P * j;
+        if (valid && b < NBT) {
+            const int base = (b / Ls) * Ls * Rr + b % Ls;
+            if constexpr (last && ACROSS) {
+#pragma unroll
+                for (int q = 0; q < Rr; q++) gb[(base + q * Ls) * n0] = cmk<C>(x[j][q].x, sg * x[j][q].y);
+            } else {
+#pragma unroll
+                for (int q = 0; q < Rr; q++) lb[(base + q * Ls) * es] = x[j][q];
+            }
+        }
+    }
+    if constexpr (!last) {
+        __syncthreads();
+        staged_line_stages<S + 1, RL, P, ACROSS>(lb, es, tw, t, valid, gb, n0, sg);
+    }
+}
+
+// lds_lines_fft for those lengths: P threads per line, lines and planes as there.
+template <typename RL, int P, int NTH, bool ACROSS, typename C, typename R>
+__device__ __forceinline__ void lds_lines_fft_staged(int nlines, int n0, int rs, int ps, const C *__restrict__ tw,
+                                                     C *__restrict__ dst, int n1, R sg) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C *buf = reinterpret_cast<C *>(smem_raw);
+    constexpr int LPI = NTH / P;
+    static_assert(LPI >= 1, "a line needs at most the block's threads");
+    const int es = ACROSS ? rs : 1;
+    const int tid = threadIdx.x;
+    const int l0 = tid % LPI;
+    int t = tid / LPI;
+    if constexpr (LPI % 64 == 0) t = __builtin_amdgcn_readfirstlane(t);
+    const bool inb = tid < LPI * P;
+    for (int first = 0; first < nlines; first += LPI) {
+        const int line = first + l0;
+        const bool valid = inb && line < nlines;
+        C *lb = buf + (!valid ? 0 : (ACROSS ? (line / n0) * ps + line % n0 : line * rs));
+        C *gb = (ACROSS && valid) ? dst + (long long)(line / n0) * n0 * n1 + line % n0 : dst;
+        staged_line_stages<0, RL, P, ACROSS>(lb, es, tw, t, valid, gb, n0, sg);
+        __syncthreads();
+    }
+}
+
+// threads per line of the staged lengths in the fused phase
+constexpr int kXyStagedThreads = 8;
+
+// Tiles the fused phase can run: every compile-time tile (register tiles through
+// lds_lines_fft, the staged tile through lds_lines_fft_staged)
+#define HMI_XY_OK(K, N, E, T, PS, ...) true,
 static const bool kXyOk[] = {HMI_BATCH_TABLE(HMI_XY_OK)};
 bool batch_xy_supported(int kid) { return kid >= 0 && kXyOk[kid]; }
+#define HMI_XY_STAGED(K, N, E, T, PS, ...) (E == 0),
+static const bool kStaged[] = {HMI_BATCH_TABLE(HMI_XY_STAGED)};
+bool batch_tile_staged(int kid) { return kid >= 0 && kStaged[kid]; }
 
 // the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
 template <typename R, int CLS, int NTH, bool ACROSS>
@@ -897,8 +975,10 @@ __device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int
                                         typename cvec<R>::type *dst, int n1, R sg) {
 #define HMI_XY_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                     \
-        if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1) \
+        if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1)   \
             lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw, dst, n1, sg); \
+        else if constexpr (E == 0 && (CLS == 0 || NTH == kXyLargeThreads))                      \
+            lds_lines_fft_staged<radices<__VA_ARGS__>, kXyStagedThreads, NTH, ACROSS>(nlines, n0, rs, ps, tw, dst, n1, sg); \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_XY_CASE)

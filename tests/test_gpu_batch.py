@@ -206,20 +206,20 @@ def test_autotune_cache_from_another_build_is_ignored(hmi, cuda, oracle, tmp_pat
 
 
 # elements per thread of the compile-time tile of each side length (0: the staged tile of
-# length 105, which the fused x-y phase does not run)
+# length 105, register class 0)
 TILE_E = {16: 16, 20: 10, 24: 12, 27: 9, 28: 14, 32: 8, 40: 10, 45: 15, 48: 12, 56: 14, 63: 21, 64: 8,
           80: 10, 96: 12, 112: 14, 128: 16, 105: 0}
 
 
 def fused_eligible(size, dtype):
-    """Whether the fused x-y phase takes a transform: both x and y lengths have register tiles;
+    """Whether the fused x-y phase takes a transform: both x and y lengths have compile-time tiles;
     register class = the larger of the two (E <= 12: 0, E <= 16: 1, else 2); complex<float>: a
     256-thread build for every class and planes of at most 8192 elements, a 512-thread build for
     planes up to 20480; complex<double>: class 0, planes up to 4096 (a plane counts n1 rows of
     n0 | 1 elements)."""
-    e0, e1 = TILE_E.get(size[0], 0), TILE_E.get(size[1], 0)
-    if e0 == 0 or e1 == 0:
+    if size[0] not in TILE_E or size[1] not in TILE_E:
         return False
+    e0, e1 = TILE_E[size[0]], TILE_E[size[1]]
     cls = max(0 if e <= 12 else (1 if e <= 16 else 2) for e in (e0, e1))
     plane = (size[0] | 1) * size[1]  # LDS rows padded to an odd length
     if dtype == "complex64":
@@ -228,11 +228,11 @@ def fused_eligible(size, dtype):
 
 
 # every build of the fused phase, z-plane groups with a remainder, columns with 3 threads per line
-# (27, 45, 63 on y), transforms left to the separate phases (105, length 1, planes beyond the
-# budget, the 21-element tile in complex<double>)
+# (27, 45, 63 on y), the staged length 105 on x and y, transforms left to the separate phases
+# (length 1, planes beyond the budget, the 14-21-element tiles in complex<double>)
 FUSED_SIZES = [(16, 16, 33), (27, 45, 5), (63, 64, 7), (20, 63, 4), (128, 128, 3), (96, 112, 2), (128, 80, 2),
                (105, 16, 4), (16, 105, 4), (63, 128, 3), (40, 24, 48), (112, 63, 2), (1, 32, 16), (64, 1, 8),
-               (45, 27, 9), (56, 28, 6), (32, 128, 5)]
+               (45, 27, 9), (56, 28, 6), (32, 128, 5), (105, 105, 3), (105, 64, 2), (20, 105, 7), (105, 128, 2)]
 
 
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])

@@ -66,7 +66,7 @@ hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, s
 // (tiles of the classes up to kXyLargeMaxClass; -1: no such build).  Every build runs without
 // VGPR spills.
 constexpr int kXySmallThreads = 256, kXyLargeThreads = 512;
-constexpr int kXySmallBudget[2] = {8192, 4096};    // elements: 64 KiB of LDS (complex<float>, <double>)
+constexpr int kXySmallBudget[2] = {10240, 5120};   // elements: 80 KiB of LDS (complex<float>, <double>)
 constexpr int kXyLargeBudget[2] = {20480, 0};      // elements: 160 KiB of LDS
 constexpr int kXyMaxClass[2] = {2, 0};             // register classes with a small build
 constexpr int kXyLargeMaxClass[2] = {2, -1};       // register classes the 512-thread build runs

@@ -244,7 +244,7 @@ int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
  * first call then runs timing launches on a stream of its own: do it outside stream capture. */
 int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable);
 /* Fused x-y phase (on by default): a transform whose x and y lengths both have compile-time
- * register tiles and whose x-y plane fits the LDS of one block (8192 complex<float> / 4096
+ * register tiles and whose x-y plane fits the LDS of one block (10240 complex<float> / 5120
  * complex<double> elements, or 20480 complex<float> elements in the 1024-thread build) runs its
  * x and y axes in one pass over its planes (one read and one write of the array instead of two);
  * the others take the separate x and y phases.  enable = 0 sends every transform through the

@@ -214,8 +214,8 @@ TILE_E = {16: 16, 20: 10, 24: 12, 27: 9, 28: 14, 32: 8, 40: 10, 45: 15, 48: 12, 
 def fused_eligible(size, dtype):
     """Whether the fused x-y phase takes a transform: both x and y lengths have compile-time tiles;
     register class = the larger of the two (E <= 12: 0, E <= 16: 1, else 2); complex<float>: a
-    256-thread build for every class and planes of at most 8192 elements, a 512-thread build for
-    planes up to 20480; complex<double>: class 0, planes up to 4096 (a plane counts n1 rows of
+    256-thread build for every class and planes of at most 10240 elements, a 512-thread build for
+    planes up to 20480; complex<double>: class 0, planes up to 5120 (a plane counts n1 rows of
     n0 | 1 elements)."""
     if size[0] not in TILE_E or size[1] not in TILE_E:
         return False
@@ -224,7 +224,7 @@ def fused_eligible(size, dtype):
     plane = (size[0] | 1) * size[1]  # LDS rows padded to an odd length
     if dtype == "complex64":
         return plane <= 20480
-    return cls == 0 and plane <= 4096
+    return cls == 0 and plane <= 5120
 
 
 # every build of the fused phase, z-plane groups with a remainder, columns with 3 threads per line

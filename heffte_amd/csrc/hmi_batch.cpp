@@ -95,6 +95,8 @@ batch_plan::~batch_plan() {
         for (auto &ax : d.rest)
             for (auto *p : ax) (void)hipFree(p);
         for (auto *p : d.xy) (void)hipFree(p);
+        for (auto *p : d.zc) (void)hipFree(p);
+        for (auto *p : d.zrest) (void)hipFree(p);
         (void)hipFree(d.plans);
         (void)hipFree(d.twoff);
         (void)hipFree(d.kid);
@@ -178,10 +180,35 @@ void batch_plan::prepare(bool dbl) {
             d.xy_lds[b] = (size_t)held[b] * elem;
         }
     }
+    // Column-block z phase: a transform whose z length has a compile-time tile of a class the
+    // 256-thread builds run; work item = zcol_width(n2, budget, n0 n1) consecutive (x, y)
+    // positions.  The others keep the tiled z phase.
+    std::vector<bool> zcol(xf_.size(), false);
+    {
+        std::vector<batch_item> zc[kBatchClasses];
+        for (int i = 0; i < (int)xf_.size(); i++) {
+            const batch_xform &x = xf_[i];
+            const int kz = kids[x.size_id[2]];
+            if (!batch_xy_supported(kz)) continue;
+            const int cls = batch_kernel_class(kz);
+            if (cls > kXyMaxClass[pr] || (long long)x.n[2] > kXySmallBudget[pr]) continue;
+            const long long plane = (long long)x.n[0] * x.n[1];
+            if (plane > 0x7fffffffLL / 2) continue;
+            const long long w = zcol_width(x.n[2], kXySmallBudget[pr], (int)plane);
+            zcol[i] = true;
+            for (long long t = 0; t * w < plane; t++) zc[cls].push_back({i, (int)t});
+        }
+        for (int c = 0; c < kBatchClasses; c++) {
+            if (zc[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
+            d.nzc[c] = (int)zc[c].size();
+            d.zc[c] = zc[c].empty() ? nullptr : upload(zc[c]);
+        }
+    }
     // work lists: every tile of every transform, one list per axis and register class; for the
-    // x and y axes also the lists without the transforms of the fused phase
+    // x and y axes also the lists without the transforms of the fused phase, for z the list
+    // without those of the column-block phase
     for (int ax = 0; ax < 3; ax++) {
-        std::vector<batch_item> items[kBatchClasses], rest[kBatchClasses];
+        std::vector<batch_item> items[kBatchClasses], rest[kBatchClasses];  // rest: z phase without zcol
         for (int i = 0; i < (int)xf_.size(); i++) {
             const batch_xform &x = xf_[i];
             const int col = (ax == 0) ? 1 : 0, b1 = 3 - ax - col;
@@ -190,7 +217,7 @@ void batch_plan::prepare(bool dbl) {
             const long long ntiles = (nlines + lines - 1) / lines;
             const int c = batch_kernel_class(kids[x.size_id[ax]]);
             for (long long t = 0; t < ntiles; t++) items[c].push_back({i, (int)t});
-            if (ax < 2 && xy_build[i] < 0)
+            if (ax < 2 ? xy_build[i] < 0 : !zcol[i])
                 for (long long t = 0; t < ntiles; t++) rest[c].push_back({i, (int)t});
         }
         for (int c = 0; c < kBatchClasses; c++) {
@@ -200,6 +227,9 @@ void batch_plan::prepare(bool dbl) {
             if (ax < 2) {
                 d.nrest[ax][c] = (int)rest[c].size();
                 d.rest[ax][c] = rest[c].empty() ? nullptr : upload(rest[c]);
+            } else {
+                d.nzrest[c] = (int)rest[c].size();
+                d.zrest[c] = rest[c].empty() ? nullptr : upload(rest[c]);
             }
         }
     }
@@ -247,11 +277,23 @@ void batch_plan::launch_phases(int dir, bool dbl, const void *in, void *out, int
         b.axis = ax;
         b.from_in = (ax == 0) ? 1 : 0;
         b.scaling = (ax == 2) ? scaling : 0;
+        if (fused && ax == 2) {
+            // column-block z phase: reads and writes the output arrays, applies the scale
+            b.xy_budget = kXySmallBudget[dbl ? 1 : 0];
+            for (int c = 0; c < kBatchClasses; c++) {
+                b.items = d.zc[c];
+                b.nitems = d.nzc[c];
+                const int nblocks = std::min(b.nitems, max_blocks);
+                if (nblocks > 0)
+                    check_hip(launch_batched_zcol(dbl, c, b, nblocks, (size_t)b.xy_budget * (dbl ? 16 : 8), s),
+                              "batched column-block z phase");
+            }
+        }
         b.xy_budget = 0;
         for (int c = 0; c < kBatchClasses; c++) {
-            const bool rest = fused && ax < 2;
-            b.items = rest ? d.rest[ax][c] : d.items[ax][c];
-            b.nitems = rest ? d.nrest[ax][c] : d.nitems[ax][c];
+            const bool rest = fused;
+            b.items = !rest ? d.items[ax][c] : (ax < 2 ? d.rest[ax][c] : d.zrest[c]);
+            b.nitems = !rest ? d.nitems[ax][c] : (ax < 2 ? d.nrest[ax][c] : d.nzrest[c]);
             // one work item per block: the hardware schedules the blocks as others retire (a
             // persistent grid of 2048 blocks walking the list was 13 % slower on the
             // 1000-transform batch); the kernel walks the list with a grid stride, so any grid

@@ -81,6 +81,12 @@ class batch_plan {
         int nxy[kBatchClasses + 1] = {};
         int xy_budget[kBatchClasses + 1] = {};
         size_t xy_lds[kBatchClasses + 1] = {};
+        // column-block z phase: one list per register class (xy_budget = kXySmallBudget), and the
+        // z work list of the transforms it does not take
+        batch_item *zc[kBatchClasses] = {};
+        int nzc[kBatchClasses] = {};
+        batch_item *zrest[kBatchClasses] = {};
+        int nzrest[kBatchClasses] = {};
         int nfused = 0;
         generic_plan *plans = nullptr;
         int *twoff = nullptr;

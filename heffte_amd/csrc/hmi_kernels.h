@@ -80,6 +80,14 @@ bool batch_tile_staged(int kid);
 hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
                              hipStream_t s);
 
+// Column-block z phase of a batch (the 256-thread builds of the fused phase, one per register
+// class up to kXyMaxClass): items (transform, block of zcol_width(n2, budget, n0 n1) consecutive
+// (x, y) positions); b.xy_budget = LDS elements per item.
+constexpr int zcol_width(int n2, int budget, int plane) {
+    return budget / n2 < 1 ? 1 : (budget / n2 < plane ? budget / n2 : plane);
+}
+hipError_t launch_batched_zcol(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s);
+
 // Precision forms of the c2c passes: plain FP32/FP64 vector ALU, or the 512-point
 // complex<float> pass on the matrix cores with split-precision operands.
 enum : int { kPrecPlain = 0, kSplitF16 = 1, kSplitBF16 = 2 };

@@ -791,6 +791,16 @@ hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, s
 
 // ------------------------------------------------------ batched, fused x-y
 
+// Where the column FFTs of the fused phases store their results: column c of block p of the
+// lines at dst + p plane + c, its rows row elements apart; each value becomes (x sc, sg y sc).
+template <typename C, typename R> struct col_store {
+    C *dst;
+    int row;
+    long long plane;
+    R sg, sc;
+    __device__ __forceinline__ C value(C v) const { return cmk<C>(v.x * sc, sg * v.y * sc); }
+};
+
 // Stockham stages of one n-point line held in LDS at lb[j * es], E elements per thread (P = n / E
 // threads per line); each exchange goes through the line's own slots.  Every thread of the block
 // calls it (the barriers are block-wide); threads without a line (valid = false) only take part
@@ -843,12 +853,11 @@ __device__ __forceinline__ void lds_line_stages(C (&v)[E], C *lb, int es, const 
 // (line l = p n0 + c at p ps + c, element stride rs).  Consecutive lanes take consecutive lines
 // at one position t of their lines: the odd row stride puts the lanes of a row pass on distinct
 // LDS banks, consecutive columns are consecutive addresses.  NTH threads, rounds of NTH / P
-// lines.  The columns' results go from registers straight to global memory: dst holds the
-// planes as n1 rows of n0 elements (consecutive lanes store consecutive columns), each value
-// times (1, sg) on the way.
+// lines.  The columns' results go from registers straight to global memory (col_store:
+// consecutive lanes store consecutive columns).
 template <typename RL, int E, int NTH, bool ACROSS, typename C, typename R>
 __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps, const C *__restrict__ tw,
-                                              C *__restrict__ dst, int n1, R sg) {
+                                              const col_store<C, R> &cs) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *buf = reinterpret_cast<C *>(smem_raw);
     constexpr int N = RL::n(), P = N / E, LPI = NTH / P;
@@ -874,9 +883,9 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
         lds_line_stages<0, RL, E, P>(v, lb, es, tw, t, valid);
         if (valid) {
             if constexpr (ACROSS) {
-                C *gb = dst + (long long)(line / n0) * n0 * n1 + line % n0;
+                C *gb = cs.dst + (line / n0) * cs.plane + line % n0;
 #pragma unroll
-                for (int e = 0; e < E; e++) gb[(t + P * e) * n0] = cmk<C>(v[e].x, sg * v[e].y);
+                for (int e = 0; e < E; e++) gb[(t + P * e) * cs.row] = cs.value(v[e]);
             } else {
 #pragma unroll
                 for (int e = 0; e < E; e++) lb[(t + P * e) * es] = v[e];
@@ -890,10 +899,10 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
 // elements-per-thread count (105 = 3 5 7): each stage reads its butterflies' inputs from the
 // line's LDS slots (thread t takes butterflies t, t + P, ... < n / R), a barrier, then writes
 // their outputs; the last stage of a column (ACROSS) writes to global memory instead (gb: the
-// column's first element, rows n0 apart, values times (1, sg)).
+// column's first element, rows cs.row apart).
 template <int S, typename RL, int P, bool ACROSS, typename C, typename R>
 __device__ __forceinline__ void staged_line_stages(C *lb, int es, const C *__restrict__ tw, int t, bool valid, C *gb,
-                                                   int n0, R sg) {
+                                                   const col_store<C, R> &cs) {
     constexpr int Rr = RL::at(S), Ls = RL::ls(S), N = RL::n(), NBT = N / Rr, NBS = (NBT + P - 1) / P;
     constexpr bool last = (S + 1 == RL::count);
     constexpr int TWO = RL::twoff(S);
@@ -920,7 +929,7 @@ __device__ __forceinline__ void staged_line_stages(C *lb, int es, const C *__res
             const int base = (b / Ls) * Ls * Rr + b % Ls;
             if constexpr (last && ACROSS) {
 #pragma unroll
-                for (int q = 0; q < Rr; q++) gb[(base + q * Ls) * n0] = cmk<C>(x[j][q].x, sg * x[j][q].y);
+                for (int q = 0; q < Rr; q++) gb[(base + q * Ls) * cs.row] = cs.value(x[j][q]);
             } else {
 #pragma unroll
                 for (int q = 0; q < Rr; q++) lb[(base + q * Ls) * es] = x[j][q];
@@ -929,14 +938,14 @@ __device__ __forceinline__ void staged_line_stages(C *lb, int es, const C *__res
     }
     if constexpr (!last) {
         __syncthreads();
-        staged_line_stages<S + 1, RL, P, ACROSS>(lb, es, tw, t, valid, gb, n0, sg);
+        staged_line_stages<S + 1, RL, P, ACROSS>(lb, es, tw, t, valid, gb, cs);
     }
 }
 
 // lds_lines_fft for those lengths: P threads per line, lines and planes as there.
 template <typename RL, int P, int NTH, bool ACROSS, typename C, typename R>
 __device__ __forceinline__ void lds_lines_fft_staged(int nlines, int n0, int rs, int ps, const C *__restrict__ tw,
-                                                     C *__restrict__ dst, int n1, R sg) {
+                                                     const col_store<C, R> &cs) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C *buf = reinterpret_cast<C *>(smem_raw);
     constexpr int LPI = NTH / P;
@@ -951,8 +960,8 @@ __device__ __forceinline__ void lds_lines_fft_staged(int nlines, int n0, int rs,
         const int line = first + l0;
         const bool valid = inb && line < nlines;
         C *lb = buf + (!valid ? 0 : (ACROSS ? (line / n0) * ps + line % n0 : line * rs));
-        C *gb = (ACROSS && valid) ? dst + (long long)(line / n0) * n0 * n1 + line % n0 : dst;
-        staged_line_stages<0, RL, P, ACROSS>(lb, es, tw, t, valid, gb, n0, sg);
+        C *gb = (ACROSS && valid) ? cs.dst + (line / n0) * cs.plane + line % n0 : cs.dst;
+        staged_line_stages<0, RL, P, ACROSS>(lb, es, tw, t, valid, gb, cs);
         __syncthreads();
     }
 }
@@ -972,13 +981,13 @@ bool batch_tile_staged(int kid) { return kid >= 0 && kStaged[kid]; }
 // the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
 template <typename R, int CLS, int NTH, bool ACROSS>
 __device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int ps, const typename cvec<R>::type *tw,
-                                        typename cvec<R>::type *dst, int n1, R sg) {
+                                        const col_store<typename cvec<R>::type, R> &cs) {
 #define HMI_XY_CASE(K, N, E, T, PS, ...)                                                        \
     case K:                                                                                     \
         if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1)   \
-            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw, dst, n1, sg); \
+            lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw, cs); \
         else if constexpr (E == 0 && (CLS == 0 || NTH == kXyLargeThreads))                      \
-            lds_lines_fft_staged<radices<__VA_ARGS__>, kXyStagedThreads, NTH, ACROSS>(nlines, n0, rs, ps, tw, dst, n1, sg); \
+            lds_lines_fft_staged<radices<__VA_ARGS__>, kXyStagedThreads, NTH, ACROSS>(nlines, n0, rs, ps, tw, cs); \
         break;
     switch (kid) {
         HMI_BATCH_TABLE(HMI_XY_CASE)
@@ -1033,8 +1042,9 @@ __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &i
     __syncthreads();
     const C *__restrict__ tw = static_cast<const C *>(b.tw);
     // x: g n1 rows of n0 points; y: g n0 columns of n1 points, stored from registers
-    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], g * n1, n0, rs, ps, tw + b.twoff[x.size_id[0]], dst, n1, sg);
-    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]], dst, n1, sg);
+    const col_store<C, R> cs{dst, n0, (long long)n0 * n1, sg, R(1)};
+    xy_axis<R, CLS, NTH, false>(b.kid[x.size_id[0]], g * n1, n0, rs, ps, tw + b.twoff[x.size_id[0]], cs);
+    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]], cs);
 }
 
 // Every build at 2 waves per SIMD (up to 256 VGPRs: no tile spills): two 256-thread blocks per
@@ -1058,6 +1068,93 @@ fft_batched_xy_large(batch_args b) {
         xy_item<float, kXyLargeMaxClass[0], kXyLargeThreads>(b, b.items[i], reinterpret_cast<float2 *>(smem_raw));
 }
 static_assert(kXyLargeMaxClass[1] < 0, "the 1024-thread build is complex<float> only");
+
+// Column-block z phase: work item = (transform, block of w <= W = min(n0 n1, budget / n2)
+// consecutive (x, y) positions), the n2-point z lines of those positions: one load of the n2
+// rows of w contiguous elements (rows n0 n1 apart in memory) into LDS rows of w, the z FFTs with
+// consecutive lanes on consecutive positions (as the column FFTs of the x-y phase), results
+// stored from registers, scaled and conjugated back for the backward transform.
+template <typename R, int CLS, int NTH>
+__device__ __forceinline__ void zcol_item(const batch_args &b, const batch_item &it, typename cvec<R>::type *buf) {
+    using C = typename cvec<R>::type;
+    const batch_xform x = b.xforms[it.xform];
+    const int plane = x.n[0] * x.n[1], n2 = x.n[2];
+    const int W = zcol_width(n2, b.xy_budget, plane);
+    const int c0 = it.tile * W;
+    const int w = (plane - c0) < W ? (plane - c0) : W;
+    const int cnt = n2 * w;
+    const C *__restrict__ src = (b.from_in ? static_cast<const C *>(b.in) + x.in_off
+                                           : static_cast<const C *>(b.out) + x.out_off) + c0;
+    C *__restrict__ dst = static_cast<C *>(b.out) + x.out_off + c0;
+    const R sg = b.conj ? R(-1) : R(1);
+    const double ntot = (double)plane * (double)n2;
+    const R sc = (R)((b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot)));
+    // element i of the block (row z = i / w, column c = i % w) sits at LDS i; walking i by NTH
+    // moves (z, c) by (NTH / w, NTH % w).  Loads go out U at a time before their LDS stores.
+    constexpr int U = 16;
+    const int dz = NTH / w, dc = NTH % w;
+    int z = (int)threadIdx.x / w, c = (int)threadIdx.x % w;
+    for (int i0 = threadIdx.x; i0 < cnt; i0 += U * NTH) {
+        C v[U];
+        int zz = z, cc = c;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (i0 + u * NTH < cnt) v[u] = src[(long long)zz * plane + cc];
+            zz += dz;
+            cc += dc;
+            if (cc >= w) cc -= w, zz++;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (i0 + u * NTH < cnt) buf[i0 + u * NTH] = cmk<C>(v[u].x, sg * v[u].y);
+        z = zz;
+        c = cc;
+    }
+    __syncthreads();
+    const col_store<C, R> cs{dst, plane, 0, sg, sc};
+    xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[2]], w, w, w, 0,
+                               static_cast<const C *>(b.tw) + b.twoff[x.size_id[2]], cs);
+}
+
+template <typename R, int CLS>
+__global__ void __launch_bounds__(kXySmallThreads)
+__attribute__((amdgpu_waves_per_eu(kXyWaves[CLS][sizeof(R) == 8])))
+fft_batched_zcol(batch_args b) {
+    using C = typename cvec<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    for (int i = blockIdx.x; i < b.nitems; i += gridDim.x)
+        zcol_item<R, CLS, kXySmallThreads>(b, b.items[i], reinterpret_cast<C *>(smem_raw));
+}
+
+template <typename R, int CLS>
+constexpr batched_fn zcol_build() {
+    if constexpr (CLS <= kXyMaxClass[sizeof(R) == 8]) return &fft_batched_zcol<R, CLS>;
+    else return nullptr;
+}
+template <int... CS>
+static const batched_fn *batched_zcol_builds(bool dbl, std::integer_sequence<int, CS...>) {
+    static const batched_fn f[2][sizeof...(CS)] = {{zcol_build<float, CS>()...}, {zcol_build<double, CS>()...}};
+    return f[dbl ? 1 : 0];
+}
+
+hipError_t launch_batched_zcol(bool dbl, int cls, const batch_args &b, int nblocks, size_t lds_bytes, hipStream_t s) {
+    static const bool configured = [] {
+        for (int d = 0; d < 2; d++)
+            for (int c = 0; c < kBatchClasses; c++) {
+                const batched_fn fn = batched_zcol_builds(d != 0, std::make_integer_sequence<int, kBatchClasses>())[c];
+                if (fn)
+                    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                              hipFuncAttributeMaxDynamicSharedMemorySize, kGenericMaxLds);
+            }
+        return true;
+    }();
+    (void)configured;
+    if (cls < 0 || cls >= kBatchClasses || cls > kXyMaxClass[dbl ? 1 : 0]) return hipErrorInvalidValue;
+    if (lds_bytes > (size_t)kGenericMaxLds) return hipErrorInvalidValue;
+    const batched_fn fn = batched_zcol_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
+    fn<<<dim3((unsigned)nblocks), dim3(kXySmallThreads), lds_bytes, s>>>(b);
+    return hipGetLastError();
+}
 
 template <typename R, int CLS>
 constexpr batched_fn xy_build() {

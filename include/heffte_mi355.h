@@ -219,12 +219,13 @@ int heffte_mi355_reshape(int elem_bytes, long long const low[3], long long const
  * heFFTe batches share one geometry, include/heffte_fft3d.h:392-402).  Each transform is a
  * c2c box of sizes[3*i .. 3*i+2] in order (0, 1, 2) stored at element offset in_offsets[i]
  * of the input array and out_offsets[i] of the output array (NULL = packed back to back;
- * out_offsets NULL = same as in_offsets).  The forward/backward calls run three phases
- * (one per axis; each one launch per register class of tile) over device work lists, replayed
- * from a captured hipGraph unless
- * heffte_mi355_batch_set_graph(batch, 0) was called; one instantiated graph is kept per
- * (direction, scale, input, output, stream, fused setting), so calls that alternate between a few argument
- * sets replay without capturing again.  scale applies per transform (1/N_i).
+ * out_offsets NULL = same as in_offsets).  The forward/backward calls run phases over device
+ * work lists (the fused x-y phase, the separate x and y phases for the transforms it does not
+ * take, then the z phase; each one launch per register class of tile), replayed from a captured
+ * hipGraph unless heffte_mi355_batch_set_graph(batch, 0) was called; one instantiated graph is
+ * kept per (direction, scale, input, output, stream, fused setting), so calls that alternate
+ * between a few argument sets replay without capturing again.  scale applies per transform
+ * (1/N_i).
  * Aliasing: the output ranges of the transforms must be disjoint (checked at creation);
  * input == output only when in_offsets[i] == out_offsets[i] for every i, otherwise the input and
  * output ranges must not overlap (checked per call, HMI_ERROR_INVALID_ARGUMENT). */
@@ -244,13 +245,15 @@ int heffte_mi355_batch_set_graph(heffte_mi355_batch batch, int enable);
  * first call then runs timing launches on a stream of its own: do it outside stream capture. */
 int heffte_mi355_batch_set_autotune(heffte_mi355_batch batch, int enable);
 /* Fused x-y phase (on by default): a transform whose x and y lengths both have compile-time
- * register tiles and whose x-y plane fits the LDS of one block (10240 complex<float> / 5120
- * complex<double> elements, or 20480 complex<float> elements in the 1024-thread build) runs its
+ * tiles and whose x-y plane fits the LDS of one block (10240 complex<float> / 5120
+ * complex<double> elements, or 20480 complex<float> elements in the 512-thread build) runs its
  * x and y axes in one pass over its planes (one read and one write of the array instead of two);
- * the others take the separate x and y phases.  enable = 0 sends every transform through the
- * separate phases.  heffte_mi355_batch_fused_count stores in *count how many transforms the
- * fused phase takes in the given precision (-1 before the first forward/backward call of that
- * precision). */
+ * the others take the separate x and y phases.  With it, a transform whose z length has a
+ * compile-time tile takes the column-block z phase (the z lines of up to 10240 / n2 consecutive
+ * (x, y) positions per block of LDS; 5120 for complex<double>).  enable = 0 sends every
+ * transform through the three separate phases.  heffte_mi355_batch_fused_count stores in *count
+ * how many transforms the fused x-y phase takes in the given precision (-1 before the first
+ * forward/backward call of that precision). */
 int heffte_mi355_batch_set_fused(heffte_mi355_batch batch, int enable);
 int heffte_mi355_batch_fused_count(heffte_mi355_batch batch, int double_precision, int *count);
 /* Elements (complex) the input / output arrays must hold. */

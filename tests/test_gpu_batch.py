@@ -229,10 +229,12 @@ def fused_eligible(size, dtype):
 
 # every build of the fused phase, z-plane groups with a remainder, columns with 3 threads per line
 # (27, 45, 63 on y), the staged length 105 on x and y, transforms left to the separate phases
-# (length 1, planes beyond the budget, the 14-21-element tiles in complex<double>)
+# (length 1, planes beyond the budget, the 14-21-element tiles in complex<double>); z lengths with
+# compile-time tiles (the column-block z phase, blocks with a remainder) and without
 FUSED_SIZES = [(16, 16, 33), (27, 45, 5), (63, 64, 7), (20, 63, 4), (128, 128, 3), (96, 112, 2), (128, 80, 2),
                (105, 16, 4), (16, 105, 4), (63, 128, 3), (40, 24, 48), (112, 63, 2), (1, 32, 16), (64, 1, 8),
-               (45, 27, 9), (56, 28, 6), (32, 128, 5), (105, 105, 3), (105, 64, 2), (20, 105, 7), (105, 128, 2)]
+               (45, 27, 9), (56, 28, 6), (32, 128, 5), (105, 105, 3), (105, 64, 2), (20, 105, 7), (105, 128, 2),
+               (16, 16, 128), (27, 45, 63), (20, 24, 105), (105, 105, 105), (128, 96, 112), (1, 32, 64), (3, 5, 80)]
 
 
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])

@@ -1020,7 +1020,7 @@ __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &i
     // element i of the group sits in LDS row i / n0 (the g n1 rows back to back), column i % n0;
     // walking i by NTH moves the pair by (NTH / n0, NTH % n0).  Loads go out U at a time before
     // their LDS stores, so that U loads per thread are in flight.
-    constexpr int U = 16;
+    constexpr int U = 24;
     const int dr = NTH / n0, dc = NTH % n0;
     const int r0 = (int)threadIdx.x / n0, c0 = (int)threadIdx.x % n0;
     {
@@ -1091,7 +1091,7 @@ __device__ __forceinline__ void zcol_item(const batch_args &b, const batch_item 
     const R sc = (R)((b.scaling == 0) ? 1.0 : (b.scaling == 1 ? 1.0 / ntot : sqrt(1.0 / ntot)));
     // element i of the block (row z = i / w, column c = i % w) sits at LDS i; walking i by NTH
     // moves (z, c) by (NTH / w, NTH % w).  Loads go out U at a time before their LDS stores.
-    constexpr int U = 16;
+    constexpr int U = 24;
     const int dz = NTH / w, dc = NTH % w;
     int z = (int)threadIdx.x / w, c = (int)threadIdx.x % w;
     for (int i0 = threadIdx.x; i0 < cnt; i0 += U * NTH) {

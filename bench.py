@@ -158,9 +158,9 @@ SMOOTH = [16, 20, 24, 27, 28, 32, 40, 45, 48, 56, 63, 64, 80, 96, 105, 112, 128]
 
 def batched_leg(heffte, torch, dev, steps):
     """1,000 independent c2c complex<float> transforms, each side drawn from the smooth sizes
-    (seed 4, sides independent), values U[-1,1]: the batched work-list kernels replayed from a
-    hipGraph vs a per-transform launch loop of the plain fft3d plans.  One step = forward +
-    backward (scale::full) of the whole batch."""
+    (seed 4, sides independent), values U[-1,1]: the batched work-list kernels (fused x-y phase,
+    column-block z phase) replayed from a hipGraph vs a per-transform launch loop of the plain
+    fft3d plans.  One step = forward + backward (scale::full) of the whole batch."""
     import numpy as np
     rng = np.random.default_rng(4)
     sizes = [tuple(int(v) for v in rng.choice(SMOOTH, 3)) for _ in range(1000)]

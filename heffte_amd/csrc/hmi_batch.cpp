@@ -144,10 +144,10 @@ void batch_plan::prepare(bool dbl) {
         plans.push_back(g);
         kids.push_back(kid);
     }
-    // Fused x-y phase: a transform whose x and y lengths both run compile-time register tiles
-    // takes the small build of the larger register class of the two when one plane fits its
-    // budget, else the 1024-thread build when a plane fits that one; work item = as many
-    // consecutive z-planes as the budget holds (at least one).
+    // Fused x-y phase: a transform whose x and y lengths both have compile-time tiles of register
+    // classes up to kXyMaxClass takes the 256-thread build when one plane fits its budget, else
+    // the 512-thread build when a plane fits that one; work item = as many consecutive z-planes
+    // as the budget holds (at least one).
     const int pr = dbl ? 1 : 0;
     std::vector<int> xy_build(xf_.size(), -1);
     {
@@ -159,9 +159,11 @@ void batch_plan::prepare(bool dbl) {
             if (!batch_xy_supported(k0) || !batch_xy_supported(k1)) continue;
             const int cls = std::max(batch_kernel_class(k0), batch_kernel_class(k1));
             const long long plane = (long long)xy_row_stride(x.n[0]) * x.n[1];  // LDS elements
-            const bool staged = batch_tile_staged(k0) || batch_tile_staged(k1);
             int b = -1;
-            if (cls <= kXyMaxClass[pr] && plane <= kXySmallBudget[pr] && !(staged && cls > 0)) b = cls;
+            // the 256-thread build is one build carrying every tile (kXyMaxClass): one launch for
+            // the items of all classes (at 2 waves per SIMD and two blocks per CU the larger
+            // register budget costs nothing; separate launches per class added their tails)
+            if (cls <= kXyMaxClass[pr] && plane <= kXySmallBudget[pr]) b = kXyMaxClass[pr];
             else if (cls <= kXyLargeMaxClass[pr] && plane <= kXyLargeBudget[pr]) b = kBatchClasses;
             if (b < 0) continue;
             xy_build[i] = b;
@@ -196,7 +198,7 @@ void batch_plan::prepare(bool dbl) {
             if (plane > 0x7fffffffLL / 2) continue;
             const long long w = zcol_width(x.n[2], kXySmallBudget[pr], (int)plane);
             zcol[i] = true;
-            for (long long t = 0; t * w < plane; t++) zc[cls].push_back({i, (int)t});
+            for (long long t = 0; t * w < plane; t++) zc[kXyMaxClass[pr]].push_back({i, (int)t});
         }
         for (int c = 0; c < kBatchClasses; c++) {
             if (zc[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");

@@ -59,30 +59,26 @@ hipError_t launch_batched(bool dbl, int cls, const batch_args &b, int nblocks, s
 // Fused x-y phase of a batch: each work item loads whole x-y planes (contiguous) of one
 // transform into LDS (n1 rows of xy_row_stride(n0) elements per plane), runs the x-axis and
 // y-axis FFTs there and stores the planes back, so the x and y axes cost one read and one write
-// of the array instead of two each.  Transforms whose x and y lengths both have register tiles
-// (batch_xy_supported) of a register class up to kXyMaxClass take it.  Builds: 256 threads with
-// planes of at most kXySmallBudget elements (one per register class, cls = the larger class of
-// the two lengths), and for complex<float> 512 threads with planes up to kXyLargeBudget elements
-// (tiles of the classes up to kXyLargeMaxClass; -1: no such build).  Every build runs without
-// VGPR spills.
+// of the array instead of two each.  Transforms whose x and y lengths both have compile-time
+// tiles (batch_xy_supported) of a register class up to kXyMaxClass take it.  Builds: 256 threads
+// with planes of at most kXySmallBudget elements (one build, cls = kXyMaxClass, carrying every
+// tile of the classes up to it), and for complex<float> 512 threads with planes up to
+// kXyLargeBudget elements (tiles of the classes up to kXyLargeMaxClass; -1: no such build).
 constexpr int kXySmallThreads = 256, kXyLargeThreads = 512;
 constexpr int kXySmallBudget[2] = {10240, 5120};   // elements: 80 KiB of LDS (complex<float>, <double>)
 constexpr int kXyLargeBudget[2] = {20480, 0};      // elements: 160 KiB of LDS
-constexpr int kXyMaxClass[2] = {2, 0};             // register classes with a small build
+constexpr int kXyMaxClass[2] = {2, 0};             // register classes of the 256-thread build
 constexpr int kXyLargeMaxClass[2] = {2, -1};       // register classes the 512-thread build runs
 // Row stride (elements) of an x-y plane of rows of n0 elements in the LDS of the fused phase: odd,
 // so that 32 lanes reading one position of 32 consecutive rows hit distinct banks.
 constexpr int xy_row_stride(int n0) { return n0 | 1; }
 bool batch_xy_supported(int kid);
-// the tile of kid is a staged tile (no register form): the 256-thread builds carry it in class 0
-// only (a transform with a staged side and a side of a higher class takes the 512-thread build)
-bool batch_tile_staged(int kid);
 hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b, int nblocks, size_t lds_bytes,
                              hipStream_t s);
 
-// Column-block z phase of a batch (the 256-thread builds of the fused phase, one per register
-// class up to kXyMaxClass): items (transform, block of zcol_width(n2, budget, n0 n1) consecutive
-// (x, y) positions); b.xy_budget = LDS elements per item.
+// Column-block z phase of a batch (256 threads, one build: cls = kXyMaxClass, carrying every
+// tile up to it): items (transform, block of zcol_width(n2, budget, n0 n1) consecutive (x, y)
+// positions); b.xy_budget = LDS elements per item.
 constexpr int zcol_width(int n2, int budget, int plane) {
     return budget / n2 < 1 ? 1 : (budget / n2 < plane ? budget / n2 : plane);
 }

@@ -974,9 +974,7 @@ constexpr int kXyStagedThreads = 8;
 #define HMI_XY_OK(K, N, E, T, PS, ...) true,
 static const bool kXyOk[] = {HMI_BATCH_TABLE(HMI_XY_OK)};
 bool batch_xy_supported(int kid) { return kid >= 0 && kXyOk[kid]; }
-#define HMI_XY_STAGED(K, N, E, T, PS, ...) (E == 0),
-static const bool kStaged[] = {HMI_BATCH_TABLE(HMI_XY_STAGED)};
-bool batch_tile_staged(int kid) { return kid >= 0 && kStaged[kid]; }
+
 
 // the lines of one axis of the planes in LDS with the tile of length id kid (class <= CLS)
 template <typename R, int CLS, int NTH, bool ACROSS>
@@ -986,7 +984,7 @@ __device__ __forceinline__ void xy_axis(int kid, int nlines, int n0, int rs, int
     case K:                                                                                     \
         if constexpr (E > 0 && batch_class_of(E) <= CLS && NTH / (N / (E > 0 ? E : 1)) >= 1)   \
             lds_lines_fft<radices<__VA_ARGS__>, (E > 0 ? E : 1), NTH, ACROSS>(nlines, n0, rs, ps, tw, cs); \
-        else if constexpr (E == 0 && (CLS == 0 || NTH == kXyLargeThreads))                      \
+        else if constexpr (E == 0)                                                              \
             lds_lines_fft_staged<radices<__VA_ARGS__>, kXyStagedThreads, NTH, ACROSS>(nlines, n0, rs, ps, tw, cs); \
         break;
     switch (kid) {
@@ -1047,8 +1045,9 @@ __device__ __forceinline__ void xy_item(const batch_args &b, const batch_item &i
     xy_axis<R, CLS, NTH, true>(b.kid[x.size_id[1]], g * n0, n0, rs, ps, tw + b.twoff[x.size_id[1]], cs);
 }
 
-// Every build at 2 waves per SIMD (up to 256 VGPRs: no tile spills): two 256-thread blocks per
-// CU with 64 KiB of LDS each, or one 512-thread block with up to 160 KiB.
+// Every build at 2 waves per SIMD (up to 256 VGPRs): two 256-thread blocks per CU with 80 KiB of
+// LDS each, or one 512-thread block with up to 160 KiB.  The launchers use only the build of
+// class kXyMaxClass, which carries every tile.
 constexpr int kXyWaves[kBatchClasses][2] = {{2, 2}, {2, 2}, {2, 2}};
 
 template <typename R, int CLS>
@@ -1128,7 +1127,7 @@ fft_batched_zcol(batch_args b) {
 
 template <typename R, int CLS>
 constexpr batched_fn zcol_build() {
-    if constexpr (CLS <= kXyMaxClass[sizeof(R) == 8]) return &fft_batched_zcol<R, CLS>;
+    if constexpr (CLS == kXyMaxClass[sizeof(R) == 8]) return &fft_batched_zcol<R, CLS>;
     else return nullptr;
 }
 template <int... CS>
@@ -1152,13 +1151,14 @@ hipError_t launch_batched_zcol(bool dbl, int cls, const batch_args &b, int nbloc
     if (cls < 0 || cls >= kBatchClasses || cls > kXyMaxClass[dbl ? 1 : 0]) return hipErrorInvalidValue;
     if (lds_bytes > (size_t)kGenericMaxLds) return hipErrorInvalidValue;
     const batched_fn fn = batched_zcol_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls];
+    if (!fn) return hipErrorInvalidValue;
     fn<<<dim3((unsigned)nblocks), dim3(kXySmallThreads), lds_bytes, s>>>(b);
     return hipGetLastError();
 }
 
 template <typename R, int CLS>
 constexpr batched_fn xy_build() {
-    if constexpr (CLS <= kXyMaxClass[sizeof(R) == 8]) return &fft_batched_xy<R, CLS>;
+    if constexpr (CLS == kXyMaxClass[sizeof(R) == 8]) return &fft_batched_xy<R, CLS>;
     else return nullptr;
 }
 template <int... CS>
@@ -1183,6 +1183,8 @@ hipError_t launch_batched_xy(bool dbl, bool large, int cls, const batch_args &b,
     }();
     (void)configured;
     if (cls < 0 || cls >= kBatchClasses || cls > (large ? kXyLargeMaxClass : kXyMaxClass)[dbl ? 1 : 0])
+        return hipErrorInvalidValue;
+    if (!large && !batched_xy_builds(dbl, std::make_integer_sequence<int, kBatchClasses>())[cls])
         return hipErrorInvalidValue;
     if (lds_bytes > (size_t)kGenericMaxLds) return hipErrorInvalidValue;
     if (large) {

@@ -265,3 +265,32 @@ def test_fused_xy_phase(hmi, cuda, oracle, dtype):
     plan.set_fused(True)
     plan.backward(ty, ty, hmi.scale.full)
     assert fc.rel_l2(fc.to_host(ty), x) <= TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", ["complex64", "complex128"])
+def test_batch_random_sides(hmi, cuda, oracle, dtype):
+    """80 transforms with sides drawn from 1..40 (primes, lengths without compile-time tiles, sides
+    of 1) mixed with 20 of table lengths: the fused x-y, column-block z and separate phases in one
+    batch, against the oracle transform by transform, forward and backward (symmetric scale)."""
+    import torch
+    rng = np.random.default_rng(12)
+    sizes = [tuple(int(v) for v in rng.integers(1, 41, 3)) for _ in range(80)]
+    sizes += [tuple(int(v) for v in rng.choice([16, 20, 24, 27, 28, 32, 40], 3)) for _ in range(20)]
+    order = rng.permutation(len(sizes))
+    sizes = [sizes[i] for i in order]
+    plan = hmi.batch_plan(sizes)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for s in sizes])[:-1]
+    x = fc.uniform_complex(plan.size_input(), 13)
+    tdt = torch.complex64 if dtype == "complex64" else torch.complex128
+    tx = torch.from_numpy(x).to(cuda).to(tdt)
+    ty = torch.empty_like(tx)
+    plan.forward(tx, ty, hmi.scale.symmetric)
+    y = fc.to_host(ty)
+    for s, off, r in zip(sizes, offsets, oracle_each(oracle, sizes, offsets, x, -1, 2)):
+        assert fc.rel_l2(y[off:off + r.size], r) <= TOL[dtype], s
+    tz = torch.empty_like(tx)
+    plan.backward(ty, tz, hmi.scale.symmetric)
+    z = fc.to_host(tz)
+    for s, off in zip(sizes, offsets):
+        n = int(np.prod(s))
+        assert fc.rel_l2(z[off:off + n], x[off:off + n]) <= TOL[dtype], s

@@ -870,10 +870,13 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
     int t = tid / LPI;
     if constexpr (LPI % 64 == 0) t = __builtin_amdgcn_readfirstlane(t);
     const bool inb = tid < LPI * P;
+    // a column line = (plane q, column r) of the planes, walked by LPI per round
+    int q = l0 / n0, r = l0 % n0;
+    const int dq = LPI / n0, dr = LPI % n0;
     for (int first = 0; first < nlines; first += LPI) {
         const int line = first + l0;
         const bool valid = inb && line < nlines;
-        C *lb = buf + (!valid ? 0 : (ACROSS ? (line / n0) * ps + line % n0 : line * rs));
+        C *lb = buf + (!valid ? 0 : (ACROSS ? q * ps + r : line * rs));
         C v[E];
         if (valid) {
 #pragma unroll
@@ -883,7 +886,7 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
         lds_line_stages<0, RL, E, P>(v, lb, es, tw, t, valid);
         if (valid) {
             if constexpr (ACROSS) {
-                C *gb = cs.dst + (line / n0) * cs.plane + line % n0;
+                C *gb = cs.dst + q * cs.plane + r;
 #pragma unroll
                 for (int e = 0; e < E; e++) gb[(t + P * e) * cs.row] = cs.value(v[e]);
             } else {
@@ -892,6 +895,9 @@ __device__ __forceinline__ void lds_lines_fft(int nlines, int n0, int rs, int ps
             }
         }
         __syncthreads();
+        q += dq;
+        r += dr;
+        if (r >= n0) r -= n0, q++;
     }
 }
 
@@ -956,13 +962,19 @@ __device__ __forceinline__ void lds_lines_fft_staged(int nlines, int n0, int rs,
     int t = tid / LPI;
     if constexpr (LPI % 64 == 0) t = __builtin_amdgcn_readfirstlane(t);
     const bool inb = tid < LPI * P;
+    // a column line = (plane q, column r) of the planes, walked by LPI per round
+    int q = l0 / n0, r = l0 % n0;
+    const int dq = LPI / n0, dr = LPI % n0;
     for (int first = 0; first < nlines; first += LPI) {
         const int line = first + l0;
         const bool valid = inb && line < nlines;
-        C *lb = buf + (!valid ? 0 : (ACROSS ? (line / n0) * ps + line % n0 : line * rs));
-        C *gb = (ACROSS && valid) ? cs.dst + (line / n0) * cs.plane + line % n0 : cs.dst;
+        C *lb = buf + (!valid ? 0 : (ACROSS ? q * ps + r : line * rs));
+        C *gb = (ACROSS && valid) ? cs.dst + q * cs.plane + r : cs.dst;
         staged_line_stages<0, RL, P, ACROSS>(lb, es, tw, t, valid, gb, cs);
         __syncthreads();
+        q += dq;
+        r += dr;
+        if (r >= n0) r -= n0, q++;
     }
 }
 

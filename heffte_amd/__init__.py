@@ -82,6 +82,8 @@ def _load():
     lib.heffte_mi355_batch_set_graph.argtypes = [plan_p, ctypes.c_int]
     lib.heffte_mi355_batch_set_autotune.argtypes = [plan_p, ctypes.c_int]
     lib.heffte_mi355_batch_set_fused.argtypes = [plan_p, ctypes.c_int]
+    lib.heffte_mi355_plan_matrix_core_passes.argtypes = [plan_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                         ctypes.POINTER(ctypes.c_int)]
     lib.heffte_mi355_batch_fused_count.argtypes = [plan_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     for f in ("heffte_mi355_batch_size_input", "heffte_mi355_batch_size_output", "heffte_mi355_batch_graph_captures"):
         getattr(lib, f).argtypes = [plan_p]
@@ -114,6 +116,7 @@ C_ABI_SYMBOLS = (
     "heffte_mi355_convolve_spectrum_c2c", "heffte_mi355_convolve_spectrum_z2z", "heffte_mi355_plan_prepare",
     "heffte_mi355_batch_graph_captures", "heffte_mi355_execute_c2c_noalloc", "heffte_mi355_plan_set_timing",
     "heffte_mi355_plan_timing_report", "heffte_mi355_batch_set_fused", "heffte_mi355_batch_fused_count",
+    "heffte_mi355_plan_matrix_core_passes",
 )
 
 
@@ -228,6 +231,14 @@ class fft_plan:
     def set_precision(self, form):
         """precision.fp32 (default), precision.split_fp16 or precision.split_bf16."""
         _check(lib.heffte_mi355_set_precision(self._h, int(form)), "set_precision")
+
+    def matrix_core_passes(self, direction=-1, in_place=False):
+        """(passes on the matrix cores, all passes) of the complex<float> transform in direction
+        -1 (forward) or +1 (backward) under the current precision form."""
+        m, t = ctypes.c_int(), ctypes.c_int()
+        _check(lib.heffte_mi355_plan_matrix_core_passes(self._h, int(direction), 1 if in_place else 0,
+                                                        ctypes.byref(m), ctypes.byref(t)), "matrix_core_passes")
+        return m.value, t.value
 
     def prepare(self, double_precision=False):
         """Uploads the device tables of every transform variant in one precision now, so that

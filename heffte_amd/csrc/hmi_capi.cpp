@@ -181,6 +181,14 @@ int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision) {
     return HMI_SUCCESS;
 }
 
+int heffte_mi355_plan_matrix_core_passes(heffte_mi355_plan plan, int direction, int in_place, int *mfma_passes,
+                                         int *total_passes) {
+    if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
+    if (!mfma_passes || !total_passes) return fail(HMI_ERROR_INVALID_ARGUMENT, "null output pointer");
+    if (direction != -1 && direction != 1) return fail(HMI_ERROR_INVALID_ARGUMENT, "direction must be -1 or +1");
+    return guarded([&] { *mfma_passes = plan->impl->matrix_core_passes(direction, in_place != 0, total_passes); });
+}
+
 int heffte_mi355_plan_prepare(heffte_mi355_plan plan, int double_precision) {
     if (!valid(plan)) return fail(HMI_ERROR_CORRUPTED_PLAN, "invalid or destroyed plan");
     return guarded([&] { plan->impl->prepare(double_precision != 0); });

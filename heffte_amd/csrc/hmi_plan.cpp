@@ -888,6 +888,21 @@ void fft_plan::execute(int dir, bool dbl, bool in_real, bool out_real, int batch
     if (tmp) check_hip(hipFreeAsync(tmp, stream_), "hipFreeAsync(workspace)");
 }
 
+int fft_plan::matrix_core_passes(int dir, bool in_place, int *total) {
+    const bool r2c = kind_ == kind::r2c, r2r = kind_ != kind::c2c && !r2c;
+    const bool in_real = r2r || (r2c && dir < 0), out_real = r2r || (r2c && dir > 0);
+    const exec_plan &ep = get_exec_plan(dir, false, in_real, out_real, in_place && !r2c);
+    int n = 0;
+    for (const pass &p : ep.passes)
+        // the same condition as the launch in execute (a convolution's fused product aside)
+        n += (precision_ != kPrecPlain && p.n == 512 && p.mode == kModeC2C && !p.identity && !p.in_real &&
+              !p.out_real && !p.herm_load && p.store_len == p.n && !p.split)
+                 ? 1
+                 : 0;
+    if (total) *total = (int)ep.passes.size();
+    return n;
+}
+
 void fft_plan::clear_timing() {
     for (hipEvent_t e : timing_events_) (void)hipEventDestroy(e);
     timing_events_.clear();

@@ -101,6 +101,9 @@ class fft_plan {
     // kPrecPlain, kSplitF16 or kSplitBF16: the form of the 512-point complex<float> c2c passes
     void set_precision(int p) { precision_ = p; }
     int precision() const { return precision_; }
+    // passes of the complex<float> transform in direction dir that run on the matrix cores
+    // under the current precision form; *total = all passes of that transform
+    int matrix_core_passes(int dir, bool in_place, int *total);
 
     // dir: -1 forward, +1 backward.  in_real/out_real select the real-valued
     // overloads of fft3d (real input on forward, real output on backward).  mul (c2c only):

@@ -111,6 +111,13 @@ int heffte_mi355_set_stream(heffte_mi355_plan plan, void *hip_stream);
 #define HMI_PRECISION_SPLIT_FP16 1
 #define HMI_PRECISION_SPLIT_BF16 2
 int heffte_mi355_set_precision(heffte_mi355_plan plan, int precision);
+/* How many passes of the plan's complex<float> transform in the given direction (-1 forward,
+ * +1 backward; in_place = input and output the same array) run on the matrix cores under the
+ * current precision form (*mfma_passes) out of all its passes (*total_passes).  The passes of
+ * other lengths keep the vector-ALU form whatever the precision form; this reports which.  No
+ * heFFTe counterpart. */
+int heffte_mi355_plan_matrix_core_passes(heffte_mi355_plan plan, int direction, int in_place, int *mfma_passes,
+                                         int *total_passes);
 
 /* Uploads the plan's device tables (twiddles, and the split-precision operands of the current
  * precision form) for every transform direction and overload, in single (double_precision = 0)

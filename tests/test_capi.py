@@ -219,3 +219,25 @@ def test_batch_aliasing_rules(hmi):
 
 def test_prepare_rejects_bad_handles(hmi):
     assert hmi.lib.heffte_mi355_plan_prepare(None, 0) == 3
+
+
+def test_matrix_core_pass_count(hmi):
+    """Which passes the split-precision forms move to the matrix cores (host planning only): the
+    three 512-point passes of the 512^3 c2c transform; none of a 1024-point axis or a 256^3 box;
+    none under FP32."""
+    b = hmi.box3d([0, 0, 0], [511, 511, 511])
+    out = hmi.box3d([0, 0, 0], [511, 511, 511], [1, 2, 0])
+    plan = hmi.fft3d(hmi.backend.mi355, b, out)
+    assert plan.matrix_core_passes(-1) == (0, 3)
+    plan.set_precision(hmi.precision.split_fp16)
+    assert plan.matrix_core_passes(-1) == (3, 3)
+    assert plan.matrix_core_passes(+1, in_place=True) == (3, 3)
+    plan.set_precision(hmi.precision.split_bf16)
+    assert plan.matrix_core_passes(+1) == (3, 3)
+    mixed = hmi.fft3d(hmi.backend.mi355, hmi.box3d([0, 0, 0], [1023, 511, 255]),
+                      hmi.box3d([0, 0, 0], [1023, 511, 255]))
+    mixed.set_precision(hmi.precision.split_fp16)
+    m, t = mixed.matrix_core_passes(-1)
+    assert (m, t) == (1, 3)
+    with pytest.raises(hmi.heffte_input_error):
+        plan.matrix_core_passes(0)

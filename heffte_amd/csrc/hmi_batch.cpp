@@ -174,7 +174,18 @@ void batch_plan::prepare(bool dbl) {
         }
         d.nfused = 0;
         for (int b : xy_build) d.nfused += b >= 0 ? 1 : 0;
+        // largest items first (the blocks take them in list order, so the launch ends on small
+        // ones: 3.925 -> 3.910 ms per pair on the 1000-transform batch)
         for (int b = 0; b <= kBatchClasses; b++) {
+            const long long budget = b == kBatchClasses ? kXyLargeBudget[pr] : kXySmallBudget[pr];
+            auto elems = [&](const batch_item &it) {
+                const batch_xform &x = xf_[it.xform];
+                const long long plane = (long long)x.n[0] * x.n[1];
+                const long long g = std::max(1LL, budget / ((long long)xy_row_stride(x.n[0]) * x.n[1]));
+                return std::min<long long>(g, x.n[2] - (long long)it.tile * g) * plane;
+            };
+            std::stable_sort(xy[b].begin(), xy[b].end(),
+                             [&](const batch_item &a, const batch_item &c) { return elems(a) > elems(c); });
             if (xy[b].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
             d.nxy[b] = (int)xy[b].size();
             d.xy[b] = xy[b].empty() ? nullptr : upload(xy[b]);
@@ -201,6 +212,14 @@ void batch_plan::prepare(bool dbl) {
             for (long long t = 0; t * w < plane; t++) zc[kXyMaxClass[pr]].push_back({i, (int)t});
         }
         for (int c = 0; c < kBatchClasses; c++) {
+            auto elems = [&](const batch_item &it) {
+                const batch_xform &x = xf_[it.xform];
+                const long long plane = (long long)x.n[0] * x.n[1];
+                const long long w = zcol_width(x.n[2], kXySmallBudget[pr], (int)plane);
+                return std::min<long long>(w, plane - (long long)it.tile * w) * x.n[2];
+            };
+            std::stable_sort(zc[c].begin(), zc[c].end(),
+                             [&](const batch_item &a, const batch_item &b) { return elems(a) > elems(b); });
             if (zc[c].size() > 0x7fffffffULL) throw std::runtime_error("batch work list too long");
             d.nzc[c] = (int)zc[c].size();
             d.zc[c] = zc[c].empty() ? nullptr : upload(zc[c]);

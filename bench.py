@@ -209,9 +209,15 @@ def batched_leg(heffte, torch, dev, steps):
     # both return the input (round trip); report the larger deviation
     err_b = (torch.linalg.vector_norm(yb - x) / torch.linalg.vector_norm(x)).item()
     err_l = (torch.linalg.vector_norm(y - x) / torch.linalg.vector_norm(x)).item()
+    # 3-pass model: one read and one write of every element per axis and direction; the fused
+    # phases move 2 passes' worth (x and y in one, z in the other)
+    model = 2 * 3 * 2 * 8 * total
+    moved = 2 * 2 * 2 * 8 * total
     return {"transforms": 1000, "elements": int(total), "distinct_shapes": len(plans),
             "ms_per_pair_batched_graph": round(ms_batched, 3), "ms_per_pair_launch_loop": round(ms_loop, 3),
             "speedup": round(ms_loop / ms_batched, 2),
+            "hbm_frac_3pass_model": round(model / (ms_batched * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "hbm_GBps_moved": round(moved / (ms_batched * 1e-3) / 1e9, 1),
             "round_trip_rel_l2": max(err_b, err_l)}
 
 

@@ -50,6 +50,21 @@ struct candidate_pass {
     bool identity;
 };
 
+// Extra cost of a pass whose load or store walks the column axis, by the width of its column
+// chunks: the tile of a long line fills the LDS with few lines (2048 complex<float> points: 8
+// lines, 64-byte chunks; 4096: 4 lines, 32 bytes), and a chunk narrower than a 128-byte cache
+// line costs HBM efficiency.  Measured on MI355X per pass against the same axis with contiguous
+// lines on both sides: +40 % at 64-byte chunks (2048 points, either precision), +150-175 % at 32
+// bytes (4096 points).  Passes without a compile-time tile are not priced here.
+double narrow_chunk_cost(const pass &p, bool dbl) {
+    int n = p.n;
+    if (p.mode < kModeDCT2 && p.real_axis && n % 2 == 0) n /= 2;
+    const fast_kernel_info *f = find_fast_kernel(n, dbl, true);
+    if (!f) return 0.0;
+    const int chunk = f->T * (dbl ? 16 : 8);
+    return chunk >= 128 ? 0.0 : chunk >= 64 ? 0.4 : chunk >= 32 ? 2.5 : 4.0;
+}
+
 // Factor pair n1 * n2 = n of a length beyond one LDS tile, both at most maxn: pairs whose two
 // lengths have compile-time kernels first, then the most balanced pair.
 bool split_length(int n, int maxn, bool dbl, int &n1, int &n2) {
@@ -307,6 +322,7 @@ exec_plan fft_plan::build(int dir, bool dbl, bool in_real, bool out_real, bool i
                     cost += 1.0;
                     if (p.load_colfast && p.store_colfast) cost += 0.15;
                     else if (p.load_colfast || p.store_colfast) cost += 0.05;
+                    if (!cp.identity && (p.load_colfast || p.store_colfast)) cost += narrow_chunk_cost(p, dbl);
                     // a copy pass runs one LDS tile per line: not along an axis too long for one
                     if (cp.identity) cost += 0.5 + (2 * dims[cp.axis] * (dbl ? 16 : 8) > kGenericMaxLds ? 100.0 : 0.0);
                     ep.passes.push_back(p);

@@ -130,12 +130,41 @@ def test_long_and_prime_lengths(hmi, cuda, oracle, dtype, size):
 @pytest.mark.parametrize("dtype", ["complex64", "complex128"])
 @pytest.mark.parametrize("size", [(1024, 20, 16), (2048, 12, 8), (16, 1024, 3)])
 def test_wide_tiles_of_long_lines(hmi, cuda, oracle, dtype, size):
-    """Passes over 1024- and 2048-point lines whose load or store walks the column axis use the
-    widest tiles (16 x 8 B or 8 x 16 B column chunks where LDS allows)."""
+    """Transposing transforms with 1024- and 2048-point axes: a 1024-point pass whose load or store
+    walks the column axis uses the widest tile (16 x 8 B or 8 x 16 B column chunks); the
+    2048-point axis (whose widest tiles have 64-byte chunks) is scheduled on contiguous lines."""
     for inorder, outorder in [((0, 1, 2), (1, 2, 0)), ((2, 1, 0), (0, 1, 2)), ((1, 0, 2), (2, 0, 1))]:
         x, y, _ = run_c2c(hmi, cuda, size, inorder, outorder, dtype, seed=6)
         ref = oracle_c2c(oracle, size, inorder, outorder, x, -1, 1.0)
         assert fc.rel_l2(y, ref) <= fc.TOL[dtype], (size, inorder, outorder)
+
+
+@pytest.mark.parametrize("dtype,size,tiles", [
+    ("complex64", (4096, 16, 8), ("radices<16, 16, 16>, 16, 1,", "radices<16, 16, 16>, 16, 4,")),
+    ("complex128", (4096, 8, 8), ("radices<16, 16, 16>, 16, 1,", "radices<16, 16, 16>, 16, 2,")),
+    ("complex64", (2048, 16, 8), ("radices<8, 16, 16>, 16, 2,", "radices<8, 16, 16>, 16, 8,"))])
+def test_long_axis_keeps_contiguous_lines(hmi, cuda, oracle, dtype, size, tiles):
+    """A transposing transform whose long axis would run few-line tiles with column chunks
+    narrower than a 128-byte line (4096 complex<float>: 4 lines of 32 bytes) is scheduled with
+    that axis on contiguous lines on both sides; the short axes carry the transposition.  The
+    schedule shows in the kernels the passes launch; the result is checked against the oracle."""
+    import torch
+    contiguous, narrow = tiles
+    n = int(np.prod(size))
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (0, 1, 2))
+    outbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], (1, 2, 0))
+    plan = hmi.fft3d(hmi.backend.mi355, inbox, outbox)
+    x = fc.uniform_complex(n, 21)
+    tx = fc.to_gpu(x, CDT[dtype])
+    ty = torch.empty(n, dtype=tx.dtype, device=cuda)
+    plan.set_timing(True)
+    plan.forward(tx, ty)
+    names = list(plan.timing())
+    plan.set_timing(False)
+    assert any(contiguous in k for k in names), names
+    assert not any(narrow in k for k in names), names
+    ref = oracle_c2c(oracle, size, (0, 1, 2), (1, 2, 0), x, -1, 1.0)
+    assert fc.rel_l2(fc.to_host(ty), ref) <= fc.TOL[dtype]
 
 
 @pytest.mark.parametrize("rdt,cdt", [(np.float32, np.complex64), (np.float64, np.complex128)])

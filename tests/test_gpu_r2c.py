@@ -80,6 +80,34 @@ def test_r2c_c2r(hmi, cuda, oracle, dtype, size, r2c_dir):
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("n0", [8192, 4096])
+def test_r2c_long_axis_across_columns(hmi, cuda, oracle, dtype, n0):
+    """r2c along an 8192- or 4096-point axis that is not the fastest of the input: the r2c pass
+    (a complex FFT of n0 / 2 points) reads its lines in column chunks of few lines (4096 points:
+    4 lines for float, 2 for double, the tiles that share cache lines numbered in groups per XCD:
+    one full group of 32 / two of 64 workgroups and a partial group over the 156 lines; 2048
+    points: 8 / 4 lines); the c2r pass of the backward transform writes the same way."""
+    import torch
+    rdt, tol = RDT[dtype]
+    size, inorder = (n0, 12, 13), (1, 0, 2)
+    inbox = hmi.box3d([0, 0, 0], [s - 1 for s in size], inorder)
+    outbox = inbox.r2c(0)
+    plan = hmi.fft3d_r2c(hmi.backend.mi355, inbox, outbox, 0)
+    x = np.random.default_rng(31).standard_normal(inbox.count())
+    tx = torch.from_numpy(x.astype(rdt)).to(cuda)
+    ty = torch.empty(outbox.count(), dtype=torch.complex64 if rdt == np.float32 else torch.complex128, device=cuda)
+    plan.forward(tx, ty)
+    bi = oracle.Box.make((0, 0, 0), [s - 1 for s in size], inorder)
+    bo = oracle.Box.make(outbox.low, outbox.high, inorder)
+    ref = oracle.fft3d_r2c(bi, bo, 0, x)
+    assert fc.rel_l2(fc.to_host(ty), ref) <= tol
+    ref_b = oracle.fft3d_c2r(bi, bo, 0, fc.to_host(ty), 1.0)
+    back = torch.empty_like(tx)
+    plan.backward(ty, back)
+    assert fc.rel_l2(fc.to_host(back), ref_b) <= tol
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_r2c_large_round_trip(hmi, cuda, dtype):
     """1024 x 1024 x 64 real (the r2c along the first, contiguous axis of the 1024 x 1024 x 512
     configuration, fewer planes): c2r(r2c(x)) / N == x and Parseval on the half spectrum."""

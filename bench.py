@@ -336,12 +336,15 @@ def reshape_leg(heffte, torch, dev, steps):
     """The bit-exact in-node reshape alone (heFFTe's transpose between pencil orders, here
     permute_kernel): 512^3 complex<float> from order (0,1,2) to (1,2,0) and to (2,1,0), out of
     place.  In the fused path this permutation costs no pass of its own; timed here as what an
-    unfused reshape would add."""
+    unfused reshape would add.  Beside it, a plain device copy of the same 1 GiB array (torch's
+    copy kernel): the rate a read + write of every element reaches without any reordering, the
+    practical ceiling the passes are compared with (below the 8 TB/s peak)."""
     n = 512
     box = heffte.box3d([0, 0, 0], [n - 1] * 3)
     x = torch.randn(n ** 3, dtype=torch.complex64, device=dev)
     y = torch.empty_like(x)
-    res = {}
+    ms = _timed(torch, lambda: y.copy_(x), steps)
+    res = {"plain_copy": {"ms": round(ms, 4), "hbm_GBps": round(2 * 8 * n ** 3 / (ms * 1e-3) / 1e9, 1)}}
     for order in ((1, 2, 0), (2, 1, 0)):
         ms = _timed(torch, lambda: heffte.reshape(box, order, x, y), steps)
         res["to_" + "".join(map(str, order))] = {"ms": round(ms, 4),

@@ -395,7 +395,11 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
 // incomplete group of 8 K workgroups keeps its own numbering.  Measured on the transposing pass
 // of a 4096-point axis: complex<float> (32-byte chunks) 1590 -> 1450 us, complex<double> 1576 ->
 // 1500 us; at 64-byte chunks the gain was not consistent (2048 points: complex<float> no change,
-// complex<double> 11 % slower), so those keep the plain numbering.
+// complex<double> 11 % slower), so those keep the plain numbering.  Wider chunks that do not
+// start on a cache line (a line or third-axis stride of the chunked side that is not a multiple
+// of 128 bytes, e.g. the 513-wide rows of an r2c half spectrum) straddle two lines, each shared
+// with the neighbouring tile: pairs of tiles go to one XCD (1024-point pass loading the 513-wide
+// rows of r2c 1024x1024x512: 1204 -> 1112 us).
 template <int K> __device__ __forceinline__ unsigned xcd_grouped_tile(unsigned b, unsigned nb) {
     constexpr unsigned G = 8 * K;
     const unsigned g = b / G;
@@ -412,7 +416,14 @@ __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     constexpr int CHUNK = T * (int)sizeof(C);
     constexpr int K = CHUNK > 32 ? 1 : 128 / CHUNK;
     __shared__ C lds[T * RS];
-    const unsigned tile = (a.load_colfast || a.store_colfast) ? xcd_grouped_tile<K>(blockIdx.x, gridDim.x) : blockIdx.x;
+    unsigned tile = blockIdx.x;
+    if (a.load_colfast || a.store_colfast) {
+        const long long es = (long long)sizeof(C);
+        const bool straddle = (a.load_colfast && ((a.in_sa * es) % 128 || (a.nb1 > 1 && (a.in_sb1 * es) % 128))) ||
+                              (a.store_colfast && ((a.out_sa * es) % 128 || (a.nb1 > 1 && (a.out_sb1 * es) % 128)));
+        if constexpr (K > 1) tile = xcd_grouped_tile<K>(blockIdx.x, gridDim.x);
+        else if (straddle) tile = xcd_grouped_tile<2>(blockIdx.x, gridDim.x);
+    }
     uniform_tile<R, RL, E, T, PS, NTM>(a, tile, lds);
 }
 

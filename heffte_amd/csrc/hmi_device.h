@@ -35,6 +35,27 @@ __device__ __forceinline__ void st_stream(double2 *p, double2 v) {
 }
 __device__ __forceinline__ void st_stream(float *p, float v) { __builtin_nontemporal_store(v, p); }
 __device__ __forceinline__ void st_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+// Numbering of the tiles of a pass whose load or store walks the column axis in chunks of T
+// lines.  Workgroups go to the 8 XCDs round robin (workgroup i to XCD i mod 8), each XCD with
+// an L2 of its own; K tiles that are neighbours along the column axis go to K workgroups of
+// one XCD dispatched one after another: workgroup 8 K g + x + 8 h (XCD x, h < K) takes tile
+// 8 K g + K x + h, and the last, incomplete group of 8 K workgroups keeps its own numbering.
+// The passes choose K so that one XCD writes (or reads) 1 KiB runs of each row: K = 8 for
+// 128-byte chunks (512^3 complex<double>, dominant pass 801 -> 750 us; complex<float> 397 ->
+// 390 us; r2c 1024x1024x512 backward 2.97 -> 2.79 ms), and for chunks of 32 bytes or less the
+// K = 128 / chunk tiles that share a cache line (4096-point transposing pass 1590 -> 1450 us);
+// 64-byte chunks group in pairs only where a chunk straddles two cache lines (a line or
+// third-axis stride of the chunked side that is not a multiple of 128 bytes; plain 64-byte
+// chunks in pairs: 2048 points complex<float> unchanged, complex<double> 11 % slower).
+template <int K> __device__ __forceinline__ unsigned xcd_grouped_tile(unsigned b, unsigned nb) {
+    static_assert(K >= 1, "a group holds at least one tile per XCD");
+    constexpr unsigned G = 8 * K;
+    const unsigned g = b / G;
+    if (K == 1 || (g + 1) * G > nb) return b;
+    const unsigned r = b % G;
+    return g * G + K * (r % 8) + r / 8;
+}
+
 // NT selects the streaming form; passes whose output fits the 256 MB Infinity Cache keep the
 // default policy so the next pass can read it from there.
 template <bool NT, typename T> __device__ __forceinline__ T ldg(const T *p) {

@@ -388,41 +388,24 @@ __device__ __forceinline__ void uniform_tile(const pass_args &a, unsigned tile, 
     }
 }
 
-// Workgroups go to the 8 XCDs round robin (workgroup i to XCD i mod 8), each XCD with an L2 of
-// its own.  When a tile's column chunk is at most a quarter of a 128-byte cache line, the K
-// tiles whose chunks share those lines go to K workgroups of one XCD dispatched one after
-// another: workgroup 8 K g + x + 8 h (XCD x, h < K) takes tile 8 K g + K x + h; the last,
-// incomplete group of 8 K workgroups keeps its own numbering.  Measured on the transposing pass
-// of a 4096-point axis: complex<float> (32-byte chunks) 1590 -> 1450 us, complex<double> 1576 ->
-// 1500 us; at 64-byte chunks the gain was not consistent (2048 points: complex<float> no change,
-// complex<double> 11 % slower), so those keep the plain numbering.  Wider chunks that do not
-// start on a cache line (a line or third-axis stride of the chunked side that is not a multiple
-// of 128 bytes, e.g. the 513-wide rows of an r2c half spectrum) straddle two lines, each shared
-// with the neighbouring tile: pairs of tiles go to one XCD (1024-point pass loading the 513-wide
-// rows of r2c 1024x1024x512: 1204 -> 1112 us).
-template <int K> __device__ __forceinline__ unsigned xcd_grouped_tile(unsigned b, unsigned nb) {
-    constexpr unsigned G = 8 * K;
-    const unsigned g = b / G;
-    if (K == 1 || (g + 1) * G > nb) return b;
-    const unsigned r = b % G;
-    return g * G + K * (r % 8) + r / 8;
-}
-
 // NTM: bit 0 = non-temporal stores, bit 1 = non-temporal loads of the pass data
 template <typename R, typename RL, int E, int T, int PS, int NTM>
 __global__ void __launch_bounds__((RL::n() / E) * T) fft_uniform(pass_args a) {
     using C = typename cvec<R>::type;
     constexpr int RS = lds_map<RL::n(), E, PS, sizeof(C)>::RS;
     constexpr int CHUNK = T * (int)sizeof(C);
-    constexpr int K = CHUNK > 32 ? 1 : 128 / CHUNK;
     __shared__ C lds[T * RS];
     unsigned tile = blockIdx.x;
     if (a.load_colfast || a.store_colfast) {
-        const long long es = (long long)sizeof(C);
-        const bool straddle = (a.load_colfast && ((a.in_sa * es) % 128 || (a.nb1 > 1 && (a.in_sb1 * es) % 128))) ||
-                              (a.store_colfast && ((a.out_sa * es) % 128 || (a.nb1 > 1 && (a.out_sb1 * es) % 128)));
-        if constexpr (K > 1) tile = xcd_grouped_tile<K>(blockIdx.x, gridDim.x);
-        else if (straddle) tile = xcd_grouped_tile<2>(blockIdx.x, gridDim.x);
+        if constexpr (CHUNK >= 128 || CHUNK <= 32) {
+            tile = xcd_grouped_tile<(CHUNK >= 1024 ? 1 : (CHUNK >= 128 ? 1024 : 128) / CHUNK)>(blockIdx.x, gridDim.x);
+        } else {
+            const long long es = (long long)sizeof(C);
+            const bool straddle =
+                (a.load_colfast && ((a.in_sa * es) % 128 || (a.nb1 > 1 && (a.in_sb1 * es) % 128))) ||
+                (a.store_colfast && ((a.out_sa * es) % 128 || (a.nb1 > 1 && (a.out_sb1 * es) % 128)));
+            if (straddle) tile = xcd_grouped_tile<2>(blockIdx.x, gridDim.x);
+        }
     }
     uniform_tile<R, RL, E, T, PS, NTM>(a, tile, lds);
 }

@@ -221,7 +221,8 @@ fft512_mfma(pass_args a) {
     vec *opCs = reinterpret_cast<vec *>(lds + kOpC);
     vec *opAs = reinterpret_cast<vec *>(lds + kOpA);
 
-    const unsigned tile = blockIdx.x;
+    // 16-line tiles: 128-byte column chunks, numbered in groups of 8 per XCD like the FP32 pass
+    const unsigned tile = (LCF || SCF) ? xcd_grouped_tile<8>(blockIdx.x, gridDim.x) : blockIdx.x;
     const unsigned nlines = (unsigned)a.nlines;
     const float2 *__restrict__ w512 = static_cast<const float2 *>(a.tw);
     const vec *__restrict__ opA = reinterpret_cast<const vec *>(w512 + 512);

@@ -26,7 +26,7 @@ METRIC = "512³ c2c fwd+bwd ms & %HBM roofline; split-FP16 vs FP32 at ≤5e-6 re
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per pass launch from rocprofv3 FETCH_SIZE / WRITE_SIZE, each scaled by the value the
 # same counter reads for a plain 1 GiB copy (profiles/fft_uniform_512_c64_hbm_traffic.md)
-MEASURED_TRAFFIC = {(512, "complex64"): (2147816896, "profiles/fft_uniform_512_c64_hbm_traffic.md")}
+MEASURED_TRAFFIC = {(512, "complex64"): (2147835904, "profiles/fft_uniform_512_c64_hbm_traffic.md")}
 
 
 def parse():
